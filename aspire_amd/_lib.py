@@ -1,0 +1,161 @@
+"""ctypes binding of libclm.so (the C ABI declared in include/clm.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if libclm.so is missing or cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclm.so")
+
+CLM_FP32, CLM_BF16_RESCORE, CLM_FP16_RESCORE, CLM_FP64_REF = 0, 1, 2, 3
+CLM_C128, CLM_C64 = 0, 1
+
+PRECISIONS = {
+    "fp32": CLM_FP32,
+    "bf16": CLM_BF16_RESCORE,
+    "fp16": CLM_FP16_RESCORE,
+    "fp64": CLM_FP64_REF,
+}
+
+# every symbol include/clm.h declares
+EXPORTS = (
+    "clm_version", "clm_default_params", "clm_num_pairs", "clm_create",
+    "clm_destroy", "clm_last_error", "clm_clmatrix", "clm_clmatrix_device",
+    "clm_assemble", "clm_shift_equations_csc",
+)
+
+
+class ClmParams(ctypes.Structure):
+    _fields_ = [
+        ("n_r", ctypes.c_int32),
+        ("n_theta", ctypes.c_int32),
+        ("K", ctypes.c_int32),
+        ("pf_dtype", ctypes.c_int32),
+        ("max_shift", ctypes.c_double),
+        ("shift_step", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("verbose", ctypes.c_int32),
+        ("tie_margin", ctypes.c_double),
+        ("variant", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ClmStats(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_int64),
+        ("cells_rescored", ctypes.c_int64),
+        ("uncertain_pairs", ctypes.c_int64),
+        ("emitted", ctypes.c_int64),
+        ("fp64_candidates", ctypes.c_int64),
+        ("fallback_pairs", ctypes.c_int64),
+        ("small_norm_lines", ctypes.c_int64),
+        ("ms_prep", ctypes.c_double),
+        ("ms_pairs", ctypes.c_double),
+        ("ms_rescore", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("variant", ctypes.c_int32),
+        ("n_eval_shifts", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load libclm.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m aspire_amd.build` "
+            "(the HIP engine has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32p, f64p, i64p = (ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64))
+    L.clm_version.restype = ctypes.c_int
+    L.clm_version.argtypes = []
+    L.clm_default_params.restype = None
+    L.clm_default_params.argtypes = [ctypes.POINTER(ClmParams)]
+    L.clm_num_pairs.restype = ctypes.c_int64
+    L.clm_num_pairs.argtypes = [ctypes.c_int32]
+    L.clm_create.restype = vp
+    L.clm_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.clm_destroy.restype = None
+    L.clm_destroy.argtypes = [vp]
+    L.clm_last_error.restype = ctypes.c_char_p
+    L.clm_last_error.argtypes = [vp]
+    L.clm_clmatrix.restype = ctypes.c_int
+    L.clm_clmatrix.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
+                               ctypes.POINTER(ClmStats)]
+    L.clm_clmatrix_device.restype = ctypes.c_int
+    L.clm_clmatrix_device.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
+                                      ctypes.POINTER(ClmStats)]
+    L.clm_assemble.restype = ctypes.c_int
+    L.clm_assemble.argtypes = [ctypes.c_int32, vp, vp, vp, vp, vp, vp]
+    L.clm_shift_equations_csc.restype = ctypes.c_int
+    L.clm_shift_equations_csc.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp, vp, vp,
+                                          ctypes.POINTER(ctypes.c_int64)]
+    _lib = L
+    return L
+
+
+class ClmError(RuntimeError):
+    pass
+
+
+class Context:
+    """Owns one clm_ctx (one HIP stream on one device)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        err = ctypes.c_int(0)
+        self._h = L.clm_create(int(device), ctypes.byref(err))
+        if not self._h:
+            raise ClmError(f"clm_create(device={device}) failed with code {err.value}")
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int):
+        if rc != 0:
+            msg = load().clm_last_error(self._h)
+            raise ClmError(f"libclm error {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().clm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx_cache: dict = {}
+
+
+def context(device: int = 0) -> Context:
+    ctx = _ctx_cache.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _ctx_cache[device] = ctx
+    return ctx
+
+
+def default_params() -> ClmParams:
+    p = ClmParams()
+    load().clm_default_params(ctypes.byref(p))
+    return p

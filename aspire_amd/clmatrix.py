@@ -1,0 +1,181 @@
+"""Host-side mirror of ASPIRE's ``cryo_clmatrix`` over libclm's C ABI.
+
+``cryo_clmatrix(pf, NK, verbose, max_shift, shift_step)`` has the reference's
+signature and returns its five outputs in the reference's form
+(``sinograms/cryo_clmatrix.m:1-3``, semantics of
+``sinograms/cryo_clmatrix_cpu.m``):
+
+* ``clstack``     K x K float64, (k1,k2) = cl1 in [1, n_theta/2],
+                  (k2,k1) = cl2 in [1, n_theta], 0 where not set;
+* ``corrstack``   K x K float64, upper triangle = 1 - max correlation (:764);
+* ``shift_equations`` scipy.sparse CSC, P x (2K+1), rows in (k1,k2) order,
+                  last column = 1-D shift (:722-772);
+* ``shift_equations_map`` K x K float64, equation number of (k1,k2);
+* ``clstack_mask`` K x K zeros (only the reference's debug mode fills it).
+
+The pair search runs on the GPU through libclm; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import logging
+
+import numpy as np
+
+from . import _lib
+
+log = logging.getLogger("aspire_amd")
+
+
+def _n_shifts(max_shift, shift_step):
+    return int(np.ceil(2 * max_shift / shift_step + 1))
+
+
+def _params(n_r, n_theta, K, dtype, max_shift, shift_step, precision, verbose, tie_margin, variant):
+    p = _lib.default_params()
+    p.n_r, p.n_theta, p.K = int(n_r), int(n_theta), int(K)
+    p.pf_dtype = dtype
+    p.max_shift = float(max_shift)
+    p.shift_step = float(shift_step)
+    p.precision = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.verbose = int(verbose)
+    p.tie_margin = float(tie_margin)
+    p.variant = int(variant)
+    return p
+
+
+def _as_matlab_pf(pf: np.ndarray):
+    """Complex n_r x n_theta x K array in column-major memory."""
+    pf = np.asarray(pf)
+    if pf.ndim == 2:
+        pf = pf[:, :, None]
+    if pf.ndim != 3:
+        raise ValueError("pf must be n_r x n_theta x K")
+    if pf.dtype == np.complex64:
+        return np.asfortranarray(pf), _lib.CLM_C64
+    return np.asfortranarray(pf.astype(np.complex128, copy=False)), _lib.CLM_C128
+
+
+def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=0,
+                     tie_margin=1e-3, variant=0, verbose=0):
+    """Per-pair records of the common-lines search.
+
+    ``pf`` is a numpy array (host path) or a torch tensor on a HIP device of
+    shape n_r x n_theta x K whose memory is column-major (e.g. the
+    ``permute(2,1,0)`` view of a contiguous K x n_theta x n_r tensor).
+    Returns ``(records, stats)``: records hold per-pair ``cl1``, ``cl2``
+    (1-based), ``j`` (0-based shift index, -1 = none), ``corr`` (best
+    correlation) and ``shift``, in equation order; numpy arrays for numpy
+    input, device tensors for tensor input.
+    """
+    ctx = _lib.context(device)
+    stats = _lib.ClmStats()
+    if type(pf).__module__.startswith("torch"):
+        import torch
+
+        if pf.dim() == 2:
+            pf = pf.unsqueeze(2)
+        n_r, n_theta, K = pf.shape
+        base = pf.permute(2, 1, 0)
+        if not base.is_contiguous():
+            base = base.contiguous()
+        if base.dtype == torch.complex64:
+            dt = _lib.CLM_C64
+        elif base.dtype == torch.complex128:
+            dt = _lib.CLM_C128
+        else:
+            raise TypeError("pf must be complex64 or complex128")
+        P = max(K * (K - 1) // 2, 0)
+        dev = base.device
+        out = dict(cl1=torch.zeros(P, dtype=torch.int32, device=dev),
+                   cl2=torch.zeros(P, dtype=torch.int32, device=dev),
+                   j=torch.zeros(P, dtype=torch.int32, device=dev),
+                   corr=torch.zeros(P, dtype=torch.float64, device=dev),
+                   shift=torch.zeros(P, dtype=torch.float64, device=dev))
+        p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant)
+        # torch's own queue must be done with pf before libclm's stream reads it
+        torch.cuda.current_stream(dev).synchronize()
+        rc = _lib.load().clm_clmatrix_device(
+            ctx.handle, ctypes.byref(p), ctypes.c_void_p(base.data_ptr()),
+            ctypes.c_void_p(out["cl1"].data_ptr()), ctypes.c_void_p(out["cl2"].data_ptr()),
+            ctypes.c_void_p(out["j"].data_ptr()), ctypes.c_void_p(out["corr"].data_ptr()),
+            ctypes.c_void_p(out["shift"].data_ptr()), ctypes.byref(stats))
+        ctx.check(rc)
+        return out, stats.as_dict()
+    pfm, dt = _as_matlab_pf(pf)
+    n_r, n_theta, K = pfm.shape
+    P = max(K * (K - 1) // 2, 0)
+    out = dict(cl1=np.zeros(P, np.int32), cl2=np.zeros(P, np.int32), j=np.zeros(P, np.int32),
+               corr=np.zeros(P, np.float64), shift=np.zeros(P, np.float64))
+    p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant)
+    rc = _lib.load().clm_clmatrix(
+        ctx.handle, ctypes.byref(p), pfm.ctypes.data_as(ctypes.c_void_p),
+        out["cl1"].ctypes.data_as(ctypes.c_void_p), out["cl2"].ctypes.data_as(ctypes.c_void_p),
+        out["j"].ctypes.data_as(ctypes.c_void_p), out["corr"].ctypes.data_as(ctypes.c_void_p),
+        out["shift"].ctypes.data_as(ctypes.c_void_p), ctypes.byref(stats))
+    ctx.check(rc)
+    return out, stats.as_dict()
+
+
+def assemble(K: int, n_theta: int, rec):
+    """Reference outputs from per-pair records (FP64, host; libclm's
+    clm_assemble and clm_shift_equations_csc)."""
+    import scipy.sparse as sp
+
+    L = _lib.load()
+    cl1 = np.ascontiguousarray(rec["cl1"], dtype=np.int32)
+    cl2 = np.ascontiguousarray(rec["cl2"], dtype=np.int32)
+    corr = np.ascontiguousarray(rec["corr"], dtype=np.float64)
+    shift = np.ascontiguousarray(rec["shift"], dtype=np.float64)
+    clstack = np.zeros((K, K), order="F")
+    corrstack = np.zeros((K, K), order="F")
+    emap = np.zeros((K, K), order="F")
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = L.clm_assemble(int(K), vp(cl1), vp(cl2), vp(corr), vp(clstack), vp(corrstack), vp(emap))
+    if rc:
+        raise _lib.ClmError(f"clm_assemble failed ({rc})")
+    P = K * (K - 1) // 2
+    nnz = ctypes.c_int64(0)
+    jc = np.zeros(2 * K + 2, np.int64)
+    rc = L.clm_shift_equations_csc(int(K), int(n_theta), vp(cl1), vp(cl2), vp(shift), vp(jc),
+                                   None, None, ctypes.byref(nnz))
+    if rc:
+        raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
+    ir = np.zeros(nnz.value, np.int64)
+    pr = np.zeros(nnz.value, np.float64)
+    rc = L.clm_shift_equations_csc(int(K), int(n_theta), vp(cl1), vp(cl2), vp(shift), vp(jc),
+                                   vp(ir), vp(pr), ctypes.byref(nnz))
+    if rc:
+        raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
+    eqs = sp.csc_matrix((pr, ir, jc), shape=(P, 2 * K + 1))
+    return clstack, corrstack, eqs, emap
+
+
+def cryo_clmatrix(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, map_filter_radius=0,
+                  ref_clmatrix=None, ref_shifts_2d=None, *, precision="bf16", device=0):
+    """Drop-in for ``cryo_clmatrix`` (sinograms/cryo_clmatrix.m:1-3).
+
+    Arguments and defaults follow cryo_clmatrix_cpu.m:123-152: ``NK`` is
+    accepted and ignored (:123), ``max_shift`` defaults to 15, ``shift_step``
+    to 1.  ``map_filter_radius`` > 0 (argument 6) is not supported by the
+    GPU engine and raises; ``ref_clmatrix``/``ref_shifts_2d`` only feed the
+    reference's debug output and are ignored.  ``precision`` selects the
+    arithmetic: "bf16" (default), "fp16", "fp32" or "fp64".
+    """
+    if map_filter_radius:
+        raise NotImplementedError("map_filter_radius > 0 is not implemented by the HIP engine")
+    if hasattr(pf, "shape") and len(pf.shape) >= 2 and pf.shape[1] % 2:
+        raise ValueError("n_theta must be even")  # cryo_clmatrix_cpu.m:84-86
+    if verbose:
+        log.info("Shift estimation parameters: max_shift=%g   shift_step=%g", max_shift, shift_step)
+    rec, stats = clmatrix_records(pf, max_shift, shift_step, precision, device, verbose=verbose)
+    if type(pf).__module__.startswith("torch"):
+        rec = {k: v.cpu().numpy() for k, v in rec.items()}
+        n_theta, K = pf.shape[1], (pf.shape[2] if pf.dim() == 3 else 1)
+    else:
+        shp = np.shape(pf)
+        n_theta, K = shp[1], (shp[2] if len(shp) == 3 else 1)
+    if stats.get("small_norm_lines"):
+        log.warning("Ray norm is close to zero for %d lines", stats["small_norm_lines"])
+    clstack, corrstack, eqs, emap = assemble(K, n_theta, rec)
+    return clstack, corrstack, eqs, emap, np.zeros((K, K))

@@ -1,0 +1,663 @@
+// clm_kernels.hip -- gfx950 kernels of the common-lines engine.
+//
+// The hot path of ASPIRE's cryo_clmatrix_cpu (sinograms/cryo_clmatrix_cpu.m:
+// 296-399) for every pair (k1,k2), every shift s_j and every pair of lines:
+//   C1 = 2 Re((P1 .* ph_j)^H P2),  C2 = 2 Re((conj(P1) .* ph_j)^H P2)
+// followed by a first-max argmax.  Here it is computed as
+//   X_e = Re(P1.*ph_e)^T Re(P2),   Y_e = Im(P1.*ph_e)^T Im(P2)
+//   C1(s_e) = 2 (X_e + Y_e),       C2(-s_e) = 2 (X_e - Y_e)
+// (exact identity: conj(P1).*ph_{-s} = conj(P1.*ph_s)), so one MFMA pass over
+// the evaluation shifts e yields both maps; on a symmetric shift grid the
+// best of the two at every position is t = X + |Y|.
+//
+// Kernels:
+//   prep_kernel      line assembly, filter, band zeroing, normalization
+//                    (:96, :277-293, cryo_raynormalize.m:17-24) -> FP64 and
+//                    FP32 half lines + MFMA operand planes.
+//   pair_kernel      persistent all-pairs MFMA contraction with a fused
+//                    max epilogue (pass 1: best cells; pass 2: every
+//                    candidate above a certified threshold).
+//   rescore_kernel   FP32 re-score of the surviving candidates, FP64
+//                    re-score of near ties, reference tie order.
+//   exact64_kernel   FP64 exhaustive search of one pair (fallback and the
+//                    CLM_FP64_REF path); the reference's own arithmetic.
+#include "clm_device.h"
+
+namespace clm {
+
+// ---------------------------------------------------------------- operands
+template <typename OpT> struct Op;
+
+template <> struct Op<__bf16> {
+    using frag = bf16x8;
+    static constexpr int KSTEP = 16;  // k per MFMA
+    __device__ static inline f32x16 mma(frag a, frag b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Op<_Float16> {
+    using frag = f16x8;
+    static constexpr int KSTEP = 16;
+    __device__ static inline f32x16 mma(frag a, frag b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Op<float> {
+    using frag = float;
+    static constexpr int KSTEP = 2;
+    __device__ static inline f32x16 mma(frag a, frag b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// -------------------------------------------------------------- preprocess
+// One thread per line (k,l).  Sequential FP64 arithmetic in the oracle's
+// order so P64 is bit-identical to the CPU restatement.
+template <typename InT>
+__global__ void prep_kernel(const InT* __restrict__ pf, Geom g,
+                            const double* __restrict__ H,  // 2n_r-1, zeroed band
+                            double2* __restrict__ P64,     // [K][L][R] or null
+                            float2* __restrict__ P32,      // [K][L][Rp]
+                            void* __restrict__ Bop,        // [K][2][Lp][Rp]
+                            int op_kind,                   // 0 f32, 1 bf16, 2 f16
+                            unsigned* __restrict__ nrm_bits,  // [K] float bits
+                            unsigned long long* __restrict__ n_small) {
+    // lines run over the padded range l < Lp so padding rows of the operand
+    // planes are written (zeros) by the same launch
+    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= (long)g.K * g.Lp) return;
+    const int k = (int)(line / g.Lp), l = (int)(line % g.Lp);
+    const int n_r = g.n_r, R = g.R, L = g.L, T = g.T, Rp = g.Rp, Lp = g.Lp;
+    auto put = [&](size_t idx, float v) {
+        if (op_kind == 0) ((float*)Bop)[idx] = v;
+        else if (op_kind == 1) ((__bf16*)Bop)[idx] = (__bf16)v;
+        else ((_Float16*)Bop)[idx] = (_Float16)v;
+    };
+    const size_t pre0 = (((size_t)k * 2 + 0) * Lp + l) * Rp;
+    const size_t pim0 = (((size_t)k * 2 + 1) * Lp + l) * Rp;
+    if (l >= L) {
+        for (int r = 0; r < Rp; r++) { put(pre0 + r, 0.f); put(pim0 + r, 0.f); }
+        return;
+    }
+    const InT* base = pf + 2 * (size_t)k * T * n_r;
+    double ss = 0.0;
+    for (int i = 0; i < 2 * n_r - 1; i++) {
+        int rr, tt;
+        if (i < n_r - 1) { rr = n_r - 1 - i; tt = L + l; }
+        else             { rr = i - (n_r - 1); tt = l; }
+        const InT* z = base + 2 * ((size_t)tt * n_r + rr);
+        double re = (double)z[0] * H[i], im = (double)z[1] * H[i];
+        ss += re * re + im * im;
+    }
+    const double nr = sqrt(ss);
+    if (nr < 1.0e-13) atomicAdd(n_small, 1ull);
+    const InT* zc = base + 2 * ((size_t)(L + l) * n_r);
+    double2* o64 = P64 ? P64 + ((size_t)k * L + l) * R : nullptr;
+    float2* o32 = P32 + ((size_t)k * L + l) * Rp;
+    double hn = 0.0;
+    for (int r = 0; r < R; r++) {
+        const int rr = n_r - 1 - r;
+        double re = ((double)zc[2 * rr] * H[r]) / nr;
+        double im = ((double)zc[2 * rr + 1] * H[r]) / nr;
+        if (o64) o64[r] = make_double2(re, im);
+        float fr = (float)re, fi = (float)im;
+        o32[r] = make_float2(fr, fi);
+        hn += re * re + im * im;
+        put(pre0 + r, fr);
+        put(pim0 + r, fi);
+    }
+    for (int r = R; r < Rp; r++) {
+        o32[r] = make_float2(0.f, 0.f);
+        put(pre0 + r, 0.f);
+        put(pim0 + r, 0.f);
+    }
+    // max half-line norm of the image, rounded up (bound input of the
+    // certified threshold).  Non-finite norms are left out: their lines
+    // give NaN correlations, which every max below ignores.
+    float hf = (float)(sqrt(hn) * (1.0 + 1.0e-6));
+    if (isfinite(hf)) atomicMax(nrm_bits + k, __float_as_uint(hf));
+}
+
+// -------------------------------------------------------------- pair kernel
+struct PairArgs {
+    Geom g;
+    const float2* P32;     // [K][L][Rp]
+    const void* Bop;       // [K][2][Lp][Rp] operand planes
+    const float2* ph32;    // [NE][Rp] (cos, sin) of the evaluation shifts
+    const float* nrm;      // [K]
+    const int4* tiles;     // {k1, k2_first | list offset, count, slot | -1}
+    int ntiles;
+    int* counter;          // next tile (persistent work queue)
+    const int4* plist;     // explicit pairs {k2, slot, u, 0}
+    PairRec* recs;         // indexed by slot
+    EmitRec* erecs;        // indexed by u (second pass)
+    int emit;              // 0: pass 1 (cells), 1: pass 2 (threshold emit)
+    int CPP;               // waves per pair (column chunks)
+    float coef;            // delta = coef * n1 * n2 + sub * (n1 + n2)
+    float sub;
+    float tau_t;           // tie margin in t units
+};
+
+template <typename OpT, int CT, int NK, bool SYM>
+__global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    using O = Op<OpT>;
+    using frag = typename O::frag;
+    const Geom g = a.g;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int W = blockDim.x >> 6;
+    const int h = lane >> 5, lc = lane & 31;
+    const int Rp = g.Rp;
+    const int RS = Rp * 2 * (int)sizeof(OpT) + 16;  // LDS row stride (bytes)
+    char* abuf0 = smem;
+    char* abuf1 = smem + 32 * RS;
+    int* s_i = (int*)(smem + 64 * RS);        // [0] tile, [1..] per-pair counters
+    float* s_pm = (float*)(s_i + 32);         // per-wave maxima
+    int* s_unc = (int*)(s_pm + 32);           // per-pair uncertain flags
+    const int pi = wave / a.CPP, chunk = wave % a.CPP;
+    const int items = 32 * (Rp / 8);
+
+    for (;;) {
+        if (tid == 0) s_i[0] = atomicAdd(a.counter, 1);
+        if (tid >= 1 && tid < 32) s_i[tid] = 0;
+        if (tid < 32) s_unc[tid] = 0;
+        __syncthreads();
+        const int t = s_i[0];
+        if (t >= a.ntiles) break;
+        const int4 tile = a.tiles[t];
+        const int k1 = tile.x;
+        const bool active = pi < tile.z;
+        int k2 = 0, slot = 0, uidx = 0;
+        if (active) {
+            if (tile.w >= 0) { k2 = tile.y + pi; slot = tile.w + pi; }
+            else { int4 pe = a.plist[tile.y + pi]; k2 = pe.x; slot = pe.y; uidx = pe.z; }
+        }
+        // ---- B operand (image k2, this wave's CT column tiles) in registers
+        frag Bre[CT][NK], Bim[CT][NK];
+        {
+            const OpT* B = (const OpT*)a.Bop;
+            const size_t plane = (size_t)g.Lp * Rp;
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                const int col = (chunk * CT + ct) * 32 + lc;
+                const OpT* br = B + ((size_t)k2 * 2 + 0) * plane + (size_t)col * Rp;
+                const OpT* bi = br + plane;
+#pragma unroll
+                for (int ks = 0; ks < NK; ks++) {
+                    if constexpr (O::KSTEP == 16) {
+                        Bre[ct][ks] = *(const frag*)(br + 16 * ks + 8 * h);
+                        Bim[ct][ks] = *(const frag*)(bi + 16 * ks + 8 * h);
+                    } else {
+                        Bre[ct][ks] = br[h * (Rp / 2) + ks];
+                        Bim[ct][ks] = bi[h * (Rp / 2) + ks];
+                    }
+                }
+            }
+        }
+        float thr = -INFINITY;
+        if (a.emit && active) thr = a.recs[slot].T;
+        float b1 = -INFINITY, b2 = -INFINITY;
+        int i1 = -1;
+
+        // ---- A-tile generation: rows (e,l1) of image k1 times the shift phase
+        auto gen = [&](int at, char* buf) {
+            for (int it = tid; it < items; it += blockDim.x) {
+                const int row = it / (Rp / 8), kc = it % (Rp / 8);
+                const int gr = at * 32 + row;
+                float re[8], im[8];
+                if (gr < g.NROWS) {
+                    const int e = gr / g.L, l1 = gr - e * g.L;
+                    const float4* p = (const float4*)(a.P32 + ((size_t)k1 * g.L + l1) * Rp + 8 * kc);
+                    const float4* q = (const float4*)(a.ph32 + (size_t)e * Rp + 8 * kc);
+                    const float sg = (!SYM && e >= g.S) ? -1.0f : 1.0f;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        float4 pv = p[u], qv = q[u];
+                        re[2 * u] = fmaf(pv.x, qv.x, -pv.y * qv.y);
+                        im[2 * u] = sg * fmaf(pv.x, qv.y, pv.y * qv.x);
+                        re[2 * u + 1] = fmaf(pv.z, qv.z, -pv.w * qv.w);
+                        im[2 * u + 1] = sg * fmaf(pv.z, qv.w, pv.w * qv.z);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) { re[u] = 0.f; im[u] = 0.f; }
+                }
+                OpT* dre = (OpT*)(buf + row * RS) + 8 * kc;
+                OpT* dim = dre + Rp;
+                if constexpr (O::KSTEP == 16) {
+                    frag vr, vi;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) { vr[u] = (OpT)re[u]; vi[u] = (OpT)im[u]; }
+                    *(frag*)dre = vr;
+                    *(frag*)dim = vi;
+                } else {
+                    ((float4*)dre)[0] = make_float4(re[0], re[1], re[2], re[3]);
+                    ((float4*)dre)[1] = make_float4(re[4], re[5], re[6], re[7]);
+                    ((float4*)dim)[0] = make_float4(im[0], im[1], im[2], im[3]);
+                    ((float4*)dim)[1] = make_float4(im[4], im[5], im[6], im[7]);
+                }
+            }
+        };
+
+        gen(0, abuf0);
+        __syncthreads();
+        for (int at = 0; at < g.NAT; at++) {
+            char* cur = (at & 1) ? abuf1 : abuf0;
+            char* nxt = (at & 1) ? abuf0 : abuf1;
+            if (at + 1 < g.NAT) gen(at + 1, nxt);
+            if (active) {
+                f32x16 X[CT], Y[CT];
+                const char* arow = cur + lc * RS;
+#pragma unroll
+                for (int ks = 0; ks < NK; ks++) {
+                    frag fa, fb;
+                    if constexpr (O::KSTEP == 16) {
+                        fa = *(const frag*)((const OpT*)arow + 16 * ks + 8 * h);
+                        fb = *(const frag*)((const OpT*)arow + Rp + 16 * ks + 8 * h);
+                    } else {
+                        fa = ((const float*)arow)[h * (Rp / 2) + ks];
+                        fb = ((const float*)arow)[Rp + h * (Rp / 2) + ks];
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < CT; ct++) {
+                        if (ks == 0) {
+                            f32x16 z = {0.f};
+                            X[ct] = O::mma(fa, Bre[ct][0], z);
+                            Y[ct] = O::mma(fb, Bim[ct][0], z);
+                        } else {
+                            X[ct] = O::mma(fa, Bre[ct][ks], X[ct]);
+                            Y[ct] = O::mma(fb, Bim[ct][ks], Y[ct]);
+                        }
+                    }
+                }
+                // ---- fused epilogue: best of both maps, cell bookkeeping
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++) {
+                    float m = -INFINITY;
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
+                        m = fmaxf(m, tv);
+                    }
+                    if (!a.emit) {
+                        const int cid = at * g.NCT + chunk * CT + ct;
+                        const bool gt = m > b1;
+                        b2 = gt ? b1 : fmaxf(b2, m);
+                        i1 = gt ? cid : i1;
+                        b1 = gt ? m : b1;
+                    } else if (m >= thr) {
+                        const int col = (chunk * CT + ct) * 32 + lc;
+#pragma unroll
+                        for (int v = 0; v < 16; v++) {
+                            const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
+                            const int row = at * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+                            if (tv >= thr && row < g.NROWS && col < g.L) {
+                                const int pos = atomicAdd(&s_i[1 + pi], 1);
+                                if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- per-pair finalisation
+        if (!a.emit) {
+            const float wm = wave_max(b1);
+            if (lane == 0) s_pm[wave] = wm;
+            __syncthreads();
+            if (active) {
+                float M = -INFINITY;
+                for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
+                const float n1 = a.nrm[k1], n2 = a.nrm[k2];
+                const float delta = a.coef * n1 * n2 + a.sub * (n1 + n2);
+                const float T = M - 2.0f * delta - a.tau_t;
+                const bool em = (i1 >= 0) && (b1 >= T);
+                const bool unc = (b2 >= T);
+                if (em) {
+                    const int pos = atomicAdd(&s_i[1 + pi], 1);
+                    if (pos < MAXC) a.recs[slot].cell[pos] = ((uint32_t)i1 << 6) | (uint32_t)lane;
+                }
+                if (unc) atomicOr(&s_unc[pi], 1);
+            }
+            __syncthreads();
+            if (active && chunk == 0 && lane == 0) {
+                float M = -INFINITY;
+                for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
+                const float n1 = a.nrm[k1], n2 = a.nrm[k2];
+                const float delta = a.coef * n1 * n2 + a.sub * (n1 + n2);
+                const int n = s_i[1 + pi];
+                PairRec* r = a.recs + slot;
+                r->M = M;
+                r->T = M - 2.0f * delta - a.tau_t;
+                r->k1 = k1;
+                r->k2 = k2;
+                r->ncells = n < MAXC ? n : MAXC;
+                r->flags = (n > MAXC ? F_OVERFLOW : 0) | (s_unc[pi] ? F_UNCERTAIN : 0) |
+                           (M == -INFINITY ? F_EMPTY : 0);
+            }
+        } else {
+            __syncthreads();
+            if (active && chunk == 0 && lane == 0) {
+                const int n = s_i[1 + pi];
+                a.erecs[uidx].n = n < MAXE ? n : MAXE;
+                a.erecs[uidx].overflow = n > MAXE;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ re-scoring
+struct RescoreArgs {
+    Geom g;
+    const float2* P32;        // [K][L][Rp]
+    const double2* P64;       // [K][L][R] (null for plain FP32)
+    const float2* ph32;       // [NE][Rp]
+    const double2* ph64;      // [S][R] phases of the reference grid
+    const double* shiftv;     // [S] s_j
+    const PairRec* recs;      // pass 1 records (by slot)
+    const EmitRec* erecs;     // pass 2 candidate lists (by u)
+    const int* ulist;         // pass 2: u -> slot
+    int n_items;              // pairs (pass 1) or uncertain pairs (pass 2)
+    int mode;                 // 0: cells of pass 1, 1: emitted candidates
+    int fp64;                 // re-score near ties in FP64
+    float tau;                // near-tie margin (correlation units)
+    int* unc_count;           // pass 1: uncertain pairs appended here
+    int* unc_list;
+    int* fb_count;            // pass 2: overflowing pairs appended here
+    int* fb_list;
+    unsigned long long* n64;  // FP64 re-scored candidates (stats)
+    int32_t* cl1; int32_t* cl2; int32_t* jidx; double* corr; double* shift1d;
+};
+
+// Order key of the reference's tie rule (shift index first, then the column
+// of [C1 C2], then the row): smaller key wins among equal values.
+__device__ inline long long tie_key(int j, int l2p, int l1, int L) {
+    return ((long long)j * (2 * L) + l2p) * L + l1;
+}
+
+__device__ inline bool better(float v, long long k, float bv, long long bk) {
+    return v > bv || (v == bv && k < bk);
+}
+__device__ inline bool better64(double v, long long k, double bv, long long bk) {
+    return v > bv || (v == bv && k < bk);
+}
+
+// C(l1, l2p) at shift j in FP64, the oracle's arithmetic and order.
+__device__ inline double entry64(const double2* P1, const double2* P2, const double2* ph, int R, bool flip) {
+    double acc = 0.0;
+    for (int r = 0; r < R; r++) {
+        const double pr = P1[r].x, pi = P1[r].y, qr = ph[r].x, qi = ph[r].y;
+        double xr, xi;
+        if (!flip) { xr = pr * qr - pi * qi; xi = pr * qi + pi * qr; }
+        else       { xr = pr * qr + pi * qi; xi = pr * qi - pi * qr; }
+        const double t = xr * P2[r].x, u = xi * P2[r].y;
+        acc = acc + (t + u);
+    }
+    return 2.0 * acc;
+}
+
+// Decode candidate q of a pair: pass-1 cells expand to 16 rows each,
+// pass-2 lists hold (row, col) directly.  row = -1 marks padding.
+__device__ inline void cand_rc(const RescoreArgs& a, const PairRec& rec, const EmitRec* er,
+                               int q, int& row, int& col) {
+    const Geom& g = a.g;
+    if (a.mode == 0) {
+        const uint32_t c = rec.cell[q >> 4];
+        const int cl = c & 63, cid = c >> 6;
+        const int at = cid / g.NCT, ct = cid % g.NCT;
+        const int ri = q & 15;
+        row = at * 32 + 8 * (ri >> 2) + 4 * (cl >> 5) + (ri & 3);
+        col = ct * 32 + (cl & 31);
+    } else {
+        const uint32_t c = er->cand[q];
+        row = (int)(c / g.Lp);
+        col = (int)(c % g.Lp);
+    }
+    if (row >= g.NROWS || col >= g.L) row = -1;
+}
+
+// FP32 value of the (up to) two candidates of row (e,l1) x column l2.
+__device__ inline void eval32(const RescoreArgs& a, int k1, int k2, int row, int col,
+                              float& v1, long long& q1, float& v2, long long& q2) {
+    const Geom& g = a.g;
+    v1 = v2 = -INFINITY;
+    q1 = q2 = 0x7fffffffffffffffLL;
+    if (row < 0) return;
+    const int e = row / g.L, l1 = row - e * g.L;
+    const float2* p = a.P32 + ((size_t)k1 * g.L + l1) * g.Rp;
+    const float2* b = a.P32 + ((size_t)k2 * g.L + col) * g.Rp;
+    const float2* ph = a.ph32 + (size_t)e * g.Rp;
+    float X = 0.f, Y = 0.f;
+    for (int r = 0; r < g.R; r++) {
+        const float2 pv = p[r], qv = ph[r], bv = b[r];
+        const float ar = fmaf(pv.x, qv.x, -pv.y * qv.y);
+        const float ai = fmaf(pv.x, qv.y, pv.y * qv.x);
+        X = fmaf(ar, bv.x, X);
+        Y = fmaf(ai, bv.y, Y);
+    }
+    if (g.sym) {
+        v1 = 2.f * (X + Y); q1 = tie_key(e, col, l1, g.L);
+        v2 = 2.f * (X - Y); q2 = tie_key(g.S - 1 - e, g.L + col, l1, g.L);
+    } else if (e < g.S) {
+        v1 = 2.f * (X + Y); q1 = tie_key(e, col, l1, g.L);
+    } else {
+        // rows e >= S carry the phase of -s_{e-S}: C2(s_{e-S}) = 2(X - Y)
+        v2 = 2.f * (X - Y); q2 = tie_key(e - g.S, g.L + col, l1, g.L);
+    }
+}
+
+__device__ inline double eval64(const RescoreArgs& a, int k1, int k2, long long key) {
+    const Geom& g = a.g;
+    const int l1 = (int)(key % g.L);
+    const int l2p = (int)((key / g.L) % (2 * g.L));
+    const int j = (int)(key / ((long long)g.L * 2 * g.L));
+    const double2* P1 = a.P64 + ((size_t)k1 * g.L + l1) * g.R;
+    const double2* P2 = a.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
+    return entry64(P1, P2, a.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
+}
+
+__device__ inline void write_result(const RescoreArgs& a, int slot, double v, long long key) {
+    const Geom& g = a.g;
+    if (v > 0.0) {
+        const int l1 = (int)(key % g.L);
+        const int l2p = (int)((key / g.L) % (2 * g.L));
+        const int j = (int)(key / ((long long)g.L * 2 * g.L));
+        if (a.cl1) a.cl1[slot] = l1 + 1;
+        if (a.cl2) a.cl2[slot] = l2p + 1;
+        if (a.jidx) a.jidx[slot] = j;
+        if (a.corr) a.corr[slot] = v;
+        if (a.shift1d) a.shift1d[slot] = a.shiftv[j];
+    } else {
+        if (a.cl1) a.cl1[slot] = 0;
+        if (a.cl2) a.cl2[slot] = 0;
+        if (a.jidx) a.jidx[slot] = -1;
+        if (a.corr) a.corr[slot] = 0.0;
+        if (a.shift1d) a.shift1d[slot] = 0.0;
+    }
+}
+
+// One wave per pair.
+__global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = blockDim.x >> 6;
+    for (int item = blockIdx.x * W + wave; item < a.n_items; item += gridDim.x * W) {
+        const int slot = a.mode == 0 ? item : a.ulist[item];
+        const PairRec rec = a.recs[slot];
+        const int k1 = rec.k1, k2 = rec.k2;
+        const EmitRec* er = a.mode ? a.erecs + item : nullptr;
+        int nrows;
+        if (a.mode == 0) {
+            if (rec.flags & F_EMPTY) {
+                if (lane == 0) write_result(a, slot, 0.0, 0);
+                continue;
+            }
+            if (rec.flags & (F_UNCERTAIN | F_OVERFLOW)) {
+                if (lane == 0) { const int u = atomicAdd(a.unc_count, 1); a.unc_list[u] = slot; }
+                continue;
+            }
+            nrows = 16 * rec.ncells;
+        } else {
+            if (er->overflow) {
+                if (lane == 0) { const int u = atomicAdd(a.fb_count, 1); a.fb_list[u] = slot; }
+                continue;
+            }
+            nrows = er->n;
+        }
+        // ---- FP32 re-score, one lane per (row, col)
+        float bv = -INFINITY;
+        long long bk = 0x7fffffffffffffffLL;
+        for (int base = 0; base < nrows; base += 64) {
+            const int q = base + lane;
+            int row = -1, col = 0;
+            if (q < nrows) cand_rc(a, rec, er, q, row, col);
+            float v1, v2;
+            long long q1, q2;
+            eval32(a, k1, k2, row, col, v1, q1, v2, q2);
+            if (better(v1, q1, bv, bk)) { bv = v1; bk = q1; }
+            if (better(v2, q2, bv, bk)) { bv = v2; bk = q2; }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ov = __shfl_xor(bv, o, 64);
+            const long long ok = __shfl_xor(bk, o, 64);
+            if (better(ov, ok, bv, bk)) { bv = ov; bk = ok; }
+        }
+        double fv = (double)bv;
+        long long fk = bk;
+        if (a.fp64 && bv > -INFINITY) {
+            // ---- FP64 re-score of every candidate within tau of the FP32 best
+            const float lim = bv - a.tau;
+            double b64 = -INFINITY;
+            long long k64 = 0x7fffffffffffffffLL;
+            int n64 = 0;
+            for (int base = 0; base < nrows; base += 64) {
+                const int q = base + lane;
+                int row = -1, col = 0;
+                if (q < nrows) cand_rc(a, rec, er, q, row, col);
+                float v1, v2;
+                long long q1, q2;
+                eval32(a, k1, k2, row, col, v1, q1, v2, q2);
+                if (v1 >= lim) {
+                    const double v = eval64(a, k1, k2, q1);
+                    n64++;
+                    if (!isnan(v) && better64(v, q1, b64, k64)) { b64 = v; k64 = q1; }
+                }
+                if (v2 >= lim) {
+                    const double v = eval64(a, k1, k2, q2);
+                    n64++;
+                    if (!isnan(v) && better64(v, q2, b64, k64)) { b64 = v; k64 = q2; }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_xor(b64, o, 64);
+                const long long ok = __shfl_xor(k64, o, 64);
+                if (better64(ov, ok, b64, k64)) { b64 = ov; k64 = ok; }
+                n64 += __shfl_xor(n64, o, 64);
+            }
+            if (lane == 0 && a.n64) atomicAdd(a.n64, (unsigned long long)n64);
+            fv = b64;
+            fk = k64;
+        }
+        if (lane == 0) write_result(a, slot, fv, fk);
+    }
+}
+
+// ------------------------------------------------------ FP64 exhaustive
+// One workgroup per pair.  Candidates are enumerated in the reference's tie
+// order q = (j*2L + l2p)*L + l1; the best (value, q) wins.
+struct Exact64Args {
+    Geom g;
+    const double2* P64;
+    const double2* ph64;   // [S][R]
+    const double* shiftv;  // [S]
+    const int* list;       // slots to compute (null: slot = blockIdx-strided 0..n-1)
+    int n;
+    int K;
+    int32_t* cl1; int32_t* cl2; int32_t* jidx; double* corr; double* shift1d;
+};
+
+__device__ inline void slot_to_pair(long long s, int K, int& k1, int& k2) {
+    // e(k1,k2) = k1*K - k1*(k1+1)/2 + (k2-k1-1)
+    double Kd = (double)K;
+    double d = (2.0 * Kd - 1.0) * (2.0 * Kd - 1.0) - 8.0 * (double)s;
+    int a = (int)floor(((2.0 * Kd - 1.0) - sqrt(d > 0 ? d : 0)) / 2.0);
+    if (a < 0) a = 0;
+    auto first = [&](long long x) { return x * K - x * (x + 1) / 2; };
+    while (a > 0 && first(a) > s) a--;
+    while (first(a + 1) <= s) a++;
+    k1 = a;
+    k2 = (int)(s - first(a)) + a + 1;
+}
+
+__global__ __launch_bounds__(256) void exact64_kernel(Exact64Args a) {
+    __shared__ double s_v[256];
+    __shared__ long long s_k[256];
+    const Geom g = a.g;
+    const long long NQ = (long long)g.S * 2 * g.L * g.L;
+    for (int it = blockIdx.x; it < a.n; it += gridDim.x) {
+        const int slot = a.list ? a.list[it] : it;
+        int k1, k2;
+        slot_to_pair(slot, a.K, k1, k2);
+        double bv = -INFINITY;
+        long long bk = 0x7fffffffffffffffLL;
+        for (long long q = threadIdx.x; q < NQ; q += blockDim.x) {
+            const int l1 = (int)(q % g.L);
+            const int l2p = (int)((q / g.L) % (2 * g.L));
+            const int j = (int)(q / ((long long)2 * g.L * g.L));
+            const double2* P1 = a.P64 + ((size_t)k1 * g.L + l1) * g.R;
+            const double2* P2 = a.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
+            const double v = entry64(P1, P2, a.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
+            if (!isnan(v) && better64(v, q, bv, bk)) { bv = v; bk = q; }
+        }
+        s_v[threadIdx.x] = bv;
+        s_k[threadIdx.x] = bk;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                if (better64(s_v[threadIdx.x + o], s_k[threadIdx.x + o], s_v[threadIdx.x], s_k[threadIdx.x])) {
+                    s_v[threadIdx.x] = s_v[threadIdx.x + o];
+                    s_k[threadIdx.x] = s_k[threadIdx.x + o];
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double v = s_v[0];
+            const long long q = s_k[0];
+            if (v > 0.0) {
+                const int l1 = (int)(q % g.L);
+                const int l2p = (int)((q / g.L) % (2 * g.L));
+                const int j = (int)(q / ((long long)2 * g.L * g.L));
+                if (a.cl1) a.cl1[slot] = l1 + 1;
+                if (a.cl2) a.cl2[slot] = l2p + 1;
+                if (a.jidx) a.jidx[slot] = j;
+                if (a.corr) a.corr[slot] = v;
+                if (a.shift1d) a.shift1d[slot] = a.shiftv[j];
+            } else {
+                if (a.cl1) a.cl1[slot] = 0;
+                if (a.cl2) a.cl2[slot] = 0;
+                if (a.jidx) a.jidx[slot] = -1;
+                if (a.corr) a.corr[slot] = 0.0;
+                if (a.shift1d) a.shift1d[slot] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void fill_i32(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace clm

@@ -1,0 +1,157 @@
+/*
+ * clm.h -- C ABI of the MI355X common-lines engine (libclm.so).
+ *
+ * The library is a drop-in for ASPIRE's
+ *   [clstack,corrstack,shift_equations,shift_equations_map,clstack_mask] =
+ *       cryo_clmatrix(pf, NK, verbose, max_shift, shift_step)
+ * (sinograms/cryo_clmatrix.m:1-3, which dispatches to
+ * sinograms/cryo_clmatrix_cpu.m:1-3 or sinograms/cryo_clmatrix_gpu.m:1-3).
+ * ASPIRE has no FFI for this path; its "operator API" is that MATLAB
+ * function.  The entry points below are what a MEX gateway (or ctypes)
+ * binds to replace it.
+ *
+ * Plain C types only: pointers, sizes, enums and POD structs.  Every call
+ * returns 0 on success or a negative CLM_E* code; the message of the last
+ * failure on a context is returned by clm_last_error().  Nothing throws
+ * across this boundary.
+ *
+ * Data layout (MATLAB's): pf is complex, interleaved (re,im), column-major
+ * n_r x n_theta x K; element (r,t,k) starts at scalar offset
+ * 2*((k*n_theta + t)*n_r + r).  Per-pair outputs are indexed by equation
+ * number e-1, pairs ordered (k1 ascending, k2 ascending), k1 < k2 -- the row
+ * order of shift_equations (cryo_clmatrix_cpu.m:722-741).
+ */
+#ifndef CLM_H
+#define CLM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLM_VERSION 10000 /* 1.0.0 */
+
+/* error codes */
+#define CLM_OK 0
+#define CLM_E_ARG (-1)      /* invalid argument (e.g. odd n_theta)       */
+#define CLM_E_HIP (-2)      /* HIP runtime failure                        */
+#define CLM_E_NOMEM (-3)    /* device or host allocation failed           */
+#define CLM_E_UNSUP (-4)    /* shape outside what the kernels support     */
+#define CLM_E_NODEV (-5)    /* no usable GPU                              */
+
+/* arithmetic of the pair contraction */
+enum clm_precision {
+    CLM_FP32 = 0,         /* FP32 MFMA contraction, FP32 argmax (plain GPU path) */
+    CLM_BF16_RESCORE = 1, /* BF16 MFMA + FP32 re-score + FP64 near-tie re-score  */
+    CLM_FP16_RESCORE = 2, /* FP16 MFMA + FP32 re-score + FP64 near-tie re-score  */
+    CLM_FP64_REF = 3      /* FP64 on every candidate, the reference's arithmetic */
+};
+
+/* element type of pf */
+enum clm_dtype {
+    CLM_C128 = 0, /* complex double (MATLAB double)  */
+    CLM_C64 = 1   /* complex float  (MATLAB single)  */
+};
+
+typedef struct clm_params {
+    int32_t n_r;         /* rows of pf                                        */
+    int32_t n_theta;     /* columns of pf; must be even (cryo_clmatrix_cpu.m:84) */
+    int32_t K;           /* number of images                                  */
+    int32_t pf_dtype;    /* enum clm_dtype                                    */
+    double max_shift;    /* cryo_clmatrix arg 4 (default 15)                  */
+    double shift_step;   /* cryo_clmatrix arg 5 (default 1)                   */
+    int32_t precision;   /* enum clm_precision                                */
+    int32_t verbose;     /* cryo_clmatrix arg 3 bitmask; bit 1 = progress log  */
+    double tie_margin;   /* FP64 re-score of candidates within this of the
+                            FP32 best (correlation units), default 1e-3       */
+    int32_t variant;     /* 0 = tuned/default kernel variant, else a variant id */
+    int32_t reserved;
+} clm_params;
+
+typedef struct clm_stats {
+    int64_t pairs;             /* pairs computed                                  */
+    int64_t cells_rescored;    /* FP32-rescored cells (16 rows x 2 maps each)     */
+    int64_t uncertain_pairs;   /* pairs sent to the exact-threshold second pass   */
+    int64_t emitted;           /* candidates emitted by the second pass           */
+    int64_t fp64_candidates;   /* candidates re-scored in FP64                    */
+    int64_t fallback_pairs;    /* pairs finished by the FP64 exhaustive kernel    */
+    int64_t small_norm_lines;  /* lines with norm < 1e-13 (cryo_raynormalize.m:20) */
+    double ms_prep;            /* preprocessing kernel                            */
+    double ms_pairs;           /* main contraction (all pairs)                    */
+    double ms_rescore;         /* re-score + second pass + fallback               */
+    double ms_total;           /* whole device job                                */
+    int32_t variant;           /* kernel variant that ran                         */
+    int32_t n_eval_shifts;     /* shifts the contraction evaluated                */
+} clm_stats;
+
+/* Library version (CLM_VERSION of the build). */
+int clm_version(void);
+
+/* Fill p with cryo_clmatrix's defaults (max_shift 15, shift_step 1,
+ * verbose 1, BF16 re-score, tie margin 1e-3). */
+void clm_default_params(clm_params *p);
+
+/* Number of pairs K(K-1)/2. */
+int64_t clm_num_pairs(int32_t K);
+
+/* Create a context on HIP device `device` (one HIP stream per context).
+ * Replaces is_gpu()/cryo_selectGPU (common/is_gpu.m:9-11,
+ * common/cryo_selectGPU.m:56).  Returns NULL and sets *err on failure. */
+typedef struct clm_ctx clm_ctx;
+clm_ctx *clm_create(int device, int *err);
+void clm_destroy(clm_ctx *ctx);
+const char *clm_last_error(const clm_ctx *ctx);
+
+/*
+ * The hot path: all K(K-1)/2 pairs of cryo_clmatrix_cpu.m:296-751
+ * (preprocessing :96,:277-293 included).
+ *
+ * Host-buffer form: pf is host memory; cl1, cl2, jidx, corr, shift1d are host
+ * arrays of clm_num_pairs(K) entries (any may be NULL to skip it):
+ *   cl1[e]   = clstack(k1,k2) in [0, n_theta/2]   (1-based, 0 = not set)
+ *   cl2[e]   = clstack(k2,k1) in [0, n_theta]
+ *   jidx[e]  = 0-based shift index of the best match, -1 if not set
+ *   corr[e]  = best correlation (before corrstack's 1-x of :764), 0 if not set
+ *   shift1d[e] = shifts_1d(k1,k2), the RHS of equation e
+ * stats may be NULL.
+ */
+int clm_clmatrix(clm_ctx *ctx, const clm_params *p, const void *pf,
+                 int32_t *cl1, int32_t *cl2, int32_t *jidx, double *corr,
+                 double *shift1d, clm_stats *stats);
+
+/* Device-buffer form of clm_clmatrix: pf and every output are device
+ * pointers (hipMalloc or torch allocations on the context's device).  Runs
+ * on the context's stream and returns once the results are in place. */
+int clm_clmatrix_device(clm_ctx *ctx, const clm_params *p, const void *d_pf,
+                        int32_t *d_cl1, int32_t *d_cl2, int32_t *d_jidx,
+                        double *d_corr, double *d_shift1d, clm_stats *stats);
+
+/*
+ * Host-side FP64 assembly of the reference outputs from per-pair records
+ * (cryo_clmatrix_cpu.m:392-399,722-741,764-772).  Dense outputs are K x K
+ * column-major doubles (MATLAB layout); any may be NULL.
+ *   clstack(k1,k2)=cl1, clstack(k2,k1)=cl2; corrstack(k1,k2)=1-corr where
+ *   corr!=0; shift_equations_map(k1,k2)=e.
+ */
+int clm_assemble(int32_t K, const int32_t *cl1, const int32_t *cl2,
+                 const double *corr, double *clstack, double *corrstack,
+                 double *eqmap);
+
+/*
+ * shift_equations as a compressed-sparse-column matrix of size
+ * P x (2K+1) (P = K(K-1)/2), the form MATLAB's mxCreateSparse takes.
+ * Explicit zeros are not stored (MATLAB sparse semantics).
+ * Call with ir = pr = NULL to get *nnz; then with arrays of that size.
+ * jc has 2K+2 entries.  Row indices are 0-based (mwIndex convention).
+ */
+int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t *cl1,
+                            const int32_t *cl2, const double *shift1d,
+                            int64_t *jc, int64_t *ir, double *pr, int64_t *nnz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLM_H */

@@ -1,0 +1,159 @@
+"""ctypes front end of the FP64 C oracle (``oracle/clm_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: the oracle is the checker for the HIP common-lines
+engine.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this module; the product path never does.
+
+The C code restates ``sinograms/cryo_clmatrix_cpu.m`` (ASPIRE) line by line;
+this module adds the MATLAB-shaped assembly of the outputs
+(``cryo_clmatrix_cpu.m:764-772``) so tests can compare whole outputs.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libclm_oracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.clmo_n_shifts.restype = ctypes.c_int
+        L.clmo_n_shifts.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.clmo_preprocess.restype = ctypes.c_int64
+        L.clmo_preprocess.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
+        L.clmo_pairs.restype = None
+        L.clmo_pairs.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, _i32p, _i32p,
+                                 ctypes.c_int64, _i32p, _i32p, _i32p, _f64p, _f64p,
+                                 ctypes.c_int]
+        L.clmo_entry.restype = ctypes.c_double
+        L.clmo_entry.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.clmo_equations.restype = None
+        L.clmo_equations.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
+                                     ctypes.c_int64, _f64p, _i32p]
+        L.clmo_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def n_shifts(max_shift: float, shift_step: float) -> int:
+    """``ceil(2*max_shift/shift_step+1)`` (cryo_clmatrix_cpu.m:140)."""
+    return int(lib().clmo_n_shifts(float(max_shift), float(shift_step)))
+
+
+def preprocess(pf: np.ndarray):
+    """Half lines ``P[k, l, r]`` (complex128) of the filtered, normalized rays.
+
+    ``pf`` is n_r x n_theta x K complex (MATLAB layout).  Returns
+    ``(P, n_small)`` where ``n_small`` counts lines with norm < 1e-13.
+    """
+    pf = np.asarray(pf)
+    if pf.ndim == 2:
+        pf = pf[:, :, None]
+    n_r, T, K = pf.shape
+    if T % 2:
+        raise ValueError("n_theta must be even")
+    buf = np.ascontiguousarray(np.asfortranarray(pf.astype(np.complex128)).ravel(order="F").view(np.float64))
+    L, R = T // 2, n_r - 1
+    P = np.empty(2 * K * L * R, dtype=np.float64)
+    n_small = lib().clmo_preprocess(buf, n_r, T, K, P)
+    return P.view(np.complex128).reshape(K, L, R), int(n_small)
+
+
+def all_pairs(K: int):
+    k1, k2 = np.triu_indices(K, 1)
+    return k1.astype(np.int32), k2.astype(np.int32)
+
+
+def pairs(P: np.ndarray, max_shift: float, shift_step: float, k1=None, k2=None,
+          nthreads: int = 0):
+    """Run the pair loop (cryo_clmatrix_cpu.m:296-399) on a list of pairs.
+
+    Returns dict with int32 ``cl1``, ``cl2`` (1-based, 0 = not set),
+    ``j`` (0-based shift index or -1), float64 ``best`` and ``shift``.
+    """
+    K, L, R = P.shape
+    if k1 is None:
+        k1, k2 = all_pairs(K)
+    k1 = np.ascontiguousarray(k1, dtype=np.int32)
+    k2 = np.ascontiguousarray(k2, dtype=np.int32)
+    n = k1.size
+    out = dict(cl1=np.zeros(n, np.int32), cl2=np.zeros(n, np.int32),
+               j=np.zeros(n, np.int32), best=np.zeros(n), shift=np.zeros(n))
+    Pf = np.ascontiguousarray(P.astype(np.complex128)).view(np.float64).ravel()
+    lib().clmo_pairs(Pf, R, L, K, float(max_shift), float(shift_step), k1, k2, n,
+                     out["cl1"], out["cl2"], out["j"], out["best"], out["shift"],
+                     int(nthreads))
+    return out
+
+
+def entry(P: np.ndarray, max_shift, shift_step, k1, k2, j, l1, l2p) -> float:
+    """C(l1, l2p) at shift index j (0-based indices; l2p >= L selects C2)."""
+    K, L, R = P.shape
+    Pf = np.ascontiguousarray(P.astype(np.complex128)).view(np.float64).ravel()
+    return float(lib().clmo_entry(Pf, R, L, float(max_shift), float(shift_step),
+                                  int(k1), int(k2), int(j), int(l1), int(l2p)))
+
+
+def equations(L: int, k1, k2, cl1, cl2):
+    """Shift-equation coefficients (n x 4) and 1-based columns (n x 4)."""
+    k1 = np.ascontiguousarray(k1, dtype=np.int32)
+    k2 = np.ascontiguousarray(k2, dtype=np.int32)
+    cl1 = np.ascontiguousarray(cl1, dtype=np.int32)
+    cl2 = np.ascontiguousarray(cl2, dtype=np.int32)
+    n = k1.size
+    coef = np.zeros(4 * n)
+    cols = np.zeros(4 * n, np.int32)
+    lib().clmo_equations(int(L), k1, k2, cl1, cl2, n, coef, cols)
+    return coef.reshape(n, 4), cols.reshape(n, 4)
+
+
+def cryo_clmatrix_cpu(pf, NK=None, verbose=0, max_shift=15, shift_step=1.0, nthreads=0):
+    """Whole-output oracle: ``[clstack, corrstack, shift_equations,
+    shift_equations_map, clstack_mask]`` as cryo_clmatrix_cpu.m returns them
+    (dense numpy / scipy.sparse CSC, MATLAB 1-based values)."""
+    import scipy.sparse as sp
+
+    P, _ = preprocess(pf)
+    K, L, R = P.shape
+    k1, k2 = all_pairs(K)
+    res = pairs(P, max_shift, shift_step, k1, k2, nthreads)
+    clstack = np.zeros((K, K))
+    corrstack = np.zeros((K, K))
+    clstack[k1, k2] = res["cl1"]
+    clstack[k2, k1] = res["cl2"]
+    corrstack[k1, k2] = res["best"]
+    nz = corrstack != 0
+    corrstack[nz] = 1.0 - corrstack[nz]                      # :764
+    coef, cols = equations(L, k1, k2, res["cl1"], res["cl2"])
+    npairs = k1.size
+    rows = np.repeat(np.arange(npairs), 4)
+    A = sp.csc_matrix((coef.ravel(), (rows, cols.ravel() - 1)), shape=(npairs, 2 * K))
+    b = sp.csc_matrix(res["shift"].reshape(-1, 1))
+    eqs = sp.hstack([A, b], format="csc")                    # :768-772
+    eqs.eliminate_zeros()
+    emap = np.zeros((K, K))
+    emap[k1, k2] = np.arange(1, npairs + 1)
+    return clstack, corrstack, eqs, emap, np.zeros((K, K))

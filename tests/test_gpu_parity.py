@@ -1,0 +1,167 @@
+"""GPU parity of libclm against the FP64 oracle (same seeded inputs).
+
+Bar (from the engine's contract): clstack agrees on >= 99.9% of pairs and
+every disagreement is a near tie within 1e-5 relative correlation (the
+oracle's own value at the GPU's choice); corr within 1e-5 relative; the
+FP64 path is bit-identical to the oracle (same arithmetic, same order).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from aspire_amd import sim
+from aspire_amd.clmatrix import clmatrix_records, cryo_clmatrix
+from tests.gpu_helpers import assert_parity, compare, oracle_records
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["fp32", "bf16", "fp16", "fp64"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def stack_c1():
+    # config-1 shape at small K: n=65, n_r=33, T=360, SNR=1, no shifts
+    return sim.make_stack(K=24, n=65, n_r=33, n_theta=360, snr=1.0, seed=1001)
+
+
+@pytest.fixture(scope="module")
+def stack_c2():
+    # config-2 shape at small K: n=89, integer shifts in [-4,4], SNR=1/8, n_r=45
+    return sim.make_stack(K=20, n=89, n_r=45, n_theta=360, snr=1 / 8, max_shift_2d=4,
+                          shift_kind="int", seed=1002)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_gpu_config1_shape(oracle, stack_c1, prec):
+    say("case config1-shape", prec)
+    pf = stack_c1.pf
+    k1, k2 = oracle.all_pairs(pf.shape[2])
+    P, ora = oracle_records(oracle, pf, 0, 1, k1, k2)
+    rec, st = clmatrix_records(pf, 0, 1, precision=prec)
+    rep = compare(oracle, P, ora, rec, 0, 1, k1, k2)
+    say(prec, rep["agree"], rep["n_disagree"], rep["corr_rel_max"], st)
+    if prec == "fp64":
+        assert rep["n_disagree"] == 0
+        assert np.array_equal(rec["corr"], ora["best"])
+    else:
+        assert_parity(rep)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_gpu_config2_shape(oracle, stack_c2, prec):
+    say("case config2-shape", prec)
+    pf = stack_c2.pf
+    k1, k2 = oracle.all_pairs(pf.shape[2])
+    P, ora = oracle_records(oracle, pf, 6, 1, k1, k2)
+    rec, st = clmatrix_records(pf, 6, 1, precision=prec)
+    rep = compare(oracle, P, ora, rec, 6, 1, k1, k2)
+    say(prec, rep["agree"], rep["n_disagree"], rep["corr_rel_max"], st)
+    if prec == "fp64":
+        assert rep["n_disagree"] == 0
+        assert np.array_equal(rec["corr"], ora["best"])
+    else:
+        assert_parity(rep)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_gpu_asymmetric_grid(oracle, stack_c2, prec):
+    """max_shift=1, step=0.3: grid -1..1.1 is not symmetric (C2 rows use -s)."""
+    say("case asym-grid", prec)
+    pf = stack_c2.pf[:, :, :10]
+    k1, k2 = oracle.all_pairs(10)
+    P, ora = oracle_records(oracle, pf, 1, 0.3, k1, k2)
+    rec, st = clmatrix_records(pf, 1, 0.3, precision=prec)
+    assert st["n_eval_shifts"] == 16
+    rep = compare(oracle, P, ora, rec, 1, 0.3, k1, k2)
+    assert_parity(rep)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_gpu_degenerate_inputs(oracle, prec):
+    """Exact ties (radially symmetric images), an all-negative pair (cl = 0)
+    and a NaN line, with the reference's tie rule."""
+    say("case degenerate", prec)
+    n_r, T = 33, 360
+    f = np.linspace(1.0, 0.2, n_r)
+    rng = np.random.default_rng(4)
+    pf = np.empty((n_r, T, 4), dtype=np.complex128, order="F")
+    pf[:, :, 0] = f[:, None]
+    pf[:, :, 1] = (0.5 * f)[:, None]
+    pf[:, :, 2] = -f[:, None]
+    half = rng.standard_normal((n_r, T // 2)) + 1j * rng.standard_normal((n_r, T // 2))
+    pf[:, : T // 2, 3] = half
+    pf[:, T // 2:, 3] = np.conj(half)
+    pf[:, 7, 3] = 0
+    pf[:, 7 + T // 2, 3] = 0
+    k1, k2 = oracle.all_pairs(4)
+    P, ora = oracle_records(oracle, pf, 2, 1, k1, k2)
+    rec, st = clmatrix_records(pf, 2, 1, precision=prec)
+    assert int(rec["cl1"][0]) == 1 and int(rec["cl2"][0]) == 1
+    assert int(rec["cl1"][1]) == 0 and int(rec["j"][1]) == -1 and rec["corr"][1] == 0
+    rep = compare(oracle, P, ora, rec, 2, 1, k1, k2)
+    assert rep["n_disagree"] == 0, rep
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_gpu_planted_lines(oracle, prec):
+    """Planted common lines (C = 1 exactly at a known line pair and shift)."""
+    from tests.test_oracle import plant, random_pf
+
+    say("case planted", prec)
+    rng = np.random.default_rng(21)
+    n_r, T, ms, step = 33, 360, 4, 0.5
+    R, L = n_r - 1, T // 2
+    pf = random_pf(rng, n_r, T, 6)
+    want = []
+    for i, (a, b) in enumerate([(0, 1), (2, 5), (3, 4)]):
+        l1, l2 = rng.integers(0, L, 2)
+        j = int(rng.integers(0, 17))
+        plant(pf, a, b, l1, l2, -ms + j * step, R, n_r, L, flipped=bool(i % 2))
+        want.append((a, b, l1 + 1, l2 + 1 + (L if i % 2 else 0), j))
+    rec, st = clmatrix_records(pf, ms, step, precision=prec)
+    K = 6
+    for a, b, c1, c2, j in want:
+        e = a * K - a * (a + 1) // 2 + (b - a - 1)
+        assert (int(rec["cl1"][e]), int(rec["cl2"][e]), int(rec["j"][e])) == (c1, c2, j)
+        assert abs(float(rec["corr"][e]) - 1.0) < 1e-5
+
+
+def test_gpu_host_and_device_entry_agree(stack_c1):
+    import torch
+
+    say("case host-vs-device")
+    pf = stack_c1.pf
+    rec_h, _ = clmatrix_records(pf, 2, 1, precision="bf16")
+    dev = torch.as_tensor(np.ascontiguousarray(pf.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
+    rec_d, _ = clmatrix_records(dev, 2, 1, precision="bf16")
+    for k in rec_h:
+        assert np.array_equal(rec_h[k], rec_d[k].cpu().numpy()), k
+
+
+def test_gpu_deterministic(stack_c2):
+    say("case determinism")
+    a, _ = clmatrix_records(stack_c2.pf, 6, 1, precision="bf16")
+    b, _ = clmatrix_records(stack_c2.pf, 6, 1, precision="bf16")
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_gpu_dropin_outputs(oracle, stack_c1):
+    """cryo_clmatrix's five outputs in the reference's form."""
+    say("case drop-in")
+    pf = stack_c1.pf
+    out = cryo_clmatrix(pf, -1, 0, 0, 1, precision="fp64")
+    ref = oracle.cryo_clmatrix_cpu(pf, max_shift=0, shift_step=1)
+    assert np.array_equal(out[0], ref[0])
+    assert np.array_equal(out[1], ref[1])
+    assert (out[2] != ref[2]).nnz == 0
+    assert np.array_equal(out[3], ref[3])
+    assert not out[4].any()
+    with pytest.raises(ValueError):
+        cryo_clmatrix(np.zeros((5, 7, 3), complex))
