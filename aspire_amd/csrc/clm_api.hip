@@ -25,6 +25,7 @@ struct DevBuf {
 struct clm_ctx {
     int device = 0;
     int n_cu = 0;
+    int lds_max = 65536;
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf P64, P32, Bop, nrm, recs, tiles, plist, ctr, ulist, erecs, fblist;
@@ -105,6 +106,7 @@ Variant choose_variant(int opk, int NCT, int Rp, int want_ct, int want_W) {
         const int ct = cts[i];
         if (want_ct && ct != want_ct) continue;
         if (opk == OPK_F32 && ct == 2 && Rp > 64) continue;
+        if (opk != OPK_F32 && ct == 3 && Rp > 64) continue;  // register budget
         const int NCTp = (NCT + ct - 1) / ct * ct;
         const int CPP = NCTp / ct;
         if (CPP > 8) continue;
@@ -122,50 +124,54 @@ Variant choose_variant(int opk, int NCT, int Rp, int want_ct, int want_W) {
 
 using PairFn = void (*)(PairArgs);
 
+template <typename OpT, int CT, int NK, bool SYM>
+PairFn pk(bool emit) {
+    return emit ? pair_kernel<OpT, CT, NK, SYM, true> : pair_kernel<OpT, CT, NK, SYM, false>;
+}
 template <typename OpT, int CT, bool SYM>
-PairFn pick_nk(int NK) {
+PairFn pick_nk(int NK, bool emit) {
     switch (NK) {
-        case 1: return pair_kernel<OpT, CT, 1, SYM>;
-        case 2: return pair_kernel<OpT, CT, 2, SYM>;
-        case 3: return pair_kernel<OpT, CT, 3, SYM>;
-        case 4: return pair_kernel<OpT, CT, 4, SYM>;
-        case 6: return pair_kernel<OpT, CT, 6, SYM>;
-        case 8: return pair_kernel<OpT, CT, 8, SYM>;
+        case 1: return pk<OpT, CT, 1, SYM>(emit);
+        case 2: return pk<OpT, CT, 2, SYM>(emit);
+        case 3: return pk<OpT, CT, 3, SYM>(emit);
+        case 4: return pk<OpT, CT, 4, SYM>(emit);
+        case 6: return pk<OpT, CT, 6, SYM>(emit);
+        case 8: return pk<OpT, CT, 8, SYM>(emit);
     }
     return nullptr;
 }
 template <int CT, bool SYM>
-PairFn pick_f32(int NK) {
+PairFn pick_f32(int NK, bool emit) {
     switch (NK) {
-        case 8: return pair_kernel<float, CT, 8, SYM>;
-        case 16: return pair_kernel<float, CT, 16, SYM>;
-        case 24: return pair_kernel<float, CT, 24, SYM>;
-        case 32: return pair_kernel<float, CT, 32, SYM>;
-        case 48: return pair_kernel<float, CT, 48, SYM>;
-        case 64: return pair_kernel<float, CT, 64, SYM>;
+        case 8: return pk<float, CT, 8, SYM>(emit);
+        case 16: return pk<float, CT, 16, SYM>(emit);
+        case 24: return pk<float, CT, 24, SYM>(emit);
+        case 32: return pk<float, CT, 32, SYM>(emit);
+        case 48: return pk<float, CT, 48, SYM>(emit);
+        case 64: return pk<float, CT, 64, SYM>(emit);
     }
     return nullptr;
 }
 template <typename OpT, bool SYM>
-PairFn pick_ct(int ct, int NK) {
+PairFn pick_ct(int ct, int NK, bool emit) {
     switch (ct) {
-        case 1: return pick_nk<OpT, 1, SYM>(NK);
-        case 2: return pick_nk<OpT, 2, SYM>(NK);
-        case 3: return pick_nk<OpT, 3, SYM>(NK);
+        case 1: return pick_nk<OpT, 1, SYM>(NK, emit);
+        case 2: return pick_nk<OpT, 2, SYM>(NK, emit);
+        case 3: return pick_nk<OpT, 3, SYM>(NK, emit);
     }
     return nullptr;
 }
 
-PairFn pick_pair_kernel(int opk, int ct, int Rp, bool sym) {
+PairFn pick_pair_kernel(int opk, int ct, int Rp, bool sym, bool emit) {
     if (opk == OPK_F32) {
         const int NK = Rp / 2;
-        if (ct == 1) return sym ? pick_f32<1, true>(NK) : pick_f32<1, false>(NK);
-        if (ct == 2 && NK <= 32) return sym ? pick_f32<2, true>(NK) : pick_f32<2, false>(NK);
+        if (ct == 1) return sym ? pick_f32<1, true>(NK, emit) : pick_f32<1, false>(NK, emit);
+        if (ct == 2 && NK <= 32) return sym ? pick_f32<2, true>(NK, emit) : pick_f32<2, false>(NK, emit);
         return nullptr;
     }
     const int NK = Rp / 16;
-    if (opk == OPK_BF16) return sym ? pick_ct<__bf16, true>(ct, NK) : pick_ct<__bf16, false>(ct, NK);
-    return sym ? pick_ct<_Float16, true>(ct, NK) : pick_ct<_Float16, false>(ct, NK);
+    if (opk == OPK_BF16) return sym ? pick_ct<__bf16, true>(ct, NK, emit) : pick_ct<__bf16, false>(ct, NK, emit);
+    return sym ? pick_ct<_Float16, true>(ct, NK, emit) : pick_ct<_Float16, false>(ct, NK, emit);
 }
 
 inline long long pair_slot(long long k1, long long k2, long long K) {
@@ -385,15 +391,20 @@ int run_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t* d_cl1
         pa.plist = nullptr;
         pa.recs = (PairRec*)c->recs.p;
         pa.erecs = nullptr;
-        pa.emit = 0;
         pa.CPP = v.CPP;
         pa.coef = (2.0f * u + u * u + (2.0f * g.Rp + 16.0f) * 0x1p-24f) * 1.01f;
         pa.sub = sub;
         pa.tau_t = use64 ? (float)(0.5 * p->tie_margin) : 0.0f;
-        PairFn fn = pick_pair_kernel(opk, v.ct, g.Rp, g.sym != 0);
-        if (!fn) return fail(c, CLM_E_UNSUP, "no kernel for ct=%d Rp=%d", v.ct, g.Rp);
-        const size_t osz_lds = opk == OPK_F32 ? 4 : 2;
-        const size_t lds = 64 * (size_t)(g.Rp * 2 * osz_lds + 16) + 384;
+        PairFn fn = pick_pair_kernel(opk, v.ct, g.Rp, g.sym != 0, false);
+        PairFn fn2 = pick_pair_kernel(opk, v.ct, g.Rp, g.sym != 0, true);
+        if (!fn || !fn2) return fail(c, CLM_E_UNSUP, "no kernel for ct=%d Rp=%d", v.ct, g.Rp);
+        const int osz_lds = opk == OPK_F32 ? 4 : 2;
+        pa.stage = pair_lds_bytes(g.Rp, osz_lds, v.W, g.NE, g.L, 1) <= (size_t)c->lds_max ? 1 : 0;
+        const size_t lds = pair_lds_bytes(g.Rp, osz_lds, v.W, g.NE, g.L, pa.stage);
+        if (lds > 65536) {
+            HIPCHK(c, hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            HIPCHK(c, hipFuncSetAttribute((const void*)fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
         int occ = 0;
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)fn, 64 * v.W, lds));
         if (occ < 1) occ = 1;
@@ -419,6 +430,8 @@ int run_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t* d_cl1
         ra.unc_list = (int*)c->ulist.p;
         ra.fb_count = (int*)(ctr + 4);
         ra.n64 = ctr + 1;
+        ra.ncell = ctr + 6;
+        ra.nemit = ctr + 7;
         ra.cl1 = d_cl1; ra.cl2 = d_cl2; ra.jidx = d_j; ra.corr = d_corr; ra.shift1d = d_shift;
         const unsigned rgrid = (unsigned)std::min<long long>((P + 3) / 4, (long long)c->n_cu * 16);
         hipLaunchKernelGGL(rescore_kernel, dim3(rgrid), dim3(256), 0, c->stream, ra);
@@ -461,9 +474,8 @@ int run_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t* d_cl1
                 pa.ntiles = (int)tl.size();
                 pa.plist = (const int4*)c->plist.p;
                 pa.erecs = (EmitRec*)c->erecs.p;
-                pa.emit = 1;
                 const unsigned g2 = (unsigned)std::min<long long>((long long)c->n_cu * occ, (long long)tl.size());
-                hipLaunchKernelGGL(fn, dim3(g2), dim3(64 * v.W), lds, c->stream, pa);
+                hipLaunchKernelGGL(fn2, dim3(g2), dim3(64 * v.W), lds, c->stream, pa);
                 hipError_t le = hipGetLastError();
                 if (le != hipSuccess) { release(tb); return fail(c, CLM_E_HIP, "pair pass 2: %s", hipGetErrorString(le)); }
                 ra.mode = 1;
@@ -523,10 +535,12 @@ int run_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t* d_cl1
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (st) {
-        unsigned long long h[2] = {0, 0};
+        unsigned long long h[8] = {0};
         HIPCHK(c, hipMemcpy(h, ctr, sizeof h, hipMemcpyDeviceToHost));
         st->small_norm_lines = (int64_t)h[0];
         st->fp64_candidates = (int64_t)h[1];
+        st->cells_rescored = (int64_t)h[6];
+        st->emitted = (int64_t)h[7];
         st->uncertain_pairs = n_unc;
         st->fallback_pairs = prec == CLM_FP64_REF ? P : n_fb;
         st->ms_prep = event_ms(c->ev[0], c->ev[1]);
@@ -574,7 +588,10 @@ clm_ctx* clm_create(int device, int* err) {
         return nullptr;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->n_cu = prop.multiProcessorCount;
+        if (prop.sharedMemPerBlock > 0) c->lds_max = (int)prop.sharedMemPerBlock;
+    }
     if (c->n_cu <= 0) c->n_cu = 256;
     for (auto& e : c->ev) (void)hipEventCreate(&e);
     if (err) *err = CLM_OK;

@@ -12,11 +12,11 @@ namespace clm {
 // Per-pair record of the first pass.  A "cell" is what one lane of a wave
 // accumulates for one 32x32 output tile: 16 rows (e,l1) x 1 column l2, each
 // row holding both maps (C1 and C2) through t = X + |Y|.
-constexpr int MAXC = 26;          // cells kept per pair (record = 128 B)
+constexpr int MAXC = 58;          // cells kept per pair (record = 256 B)
 constexpr int MAXE = 1024;        // candidates kept per pair in the second pass
 
-constexpr int F_UNCERTAIN = 1;    // some lane's second-best cell reached T
-constexpr int F_OVERFLOW = 2;     // more than MAXC cells reached T
+constexpr int F_UNCERTAIN = 1;    // a wave's cell list overflowed: a cell may be missing
+constexpr int F_OVERFLOW = 2;     // more than MAXC cells reached the final threshold
 constexpr int F_EMPTY = 4;        // no finite candidate at all
 
 struct alignas(16) PairRec {
@@ -27,7 +27,7 @@ struct alignas(16) PairRec {
     int32_t ncells;
     uint32_t cell[MAXC];  // (cell_id << 6) | lane, cell_id = at*NCT + ct
 };
-static_assert(sizeof(PairRec) == 128, "PairRec layout");
+static_assert(sizeof(PairRec) == 256, "PairRec layout");
 
 struct EmitRec {
     int32_t n;

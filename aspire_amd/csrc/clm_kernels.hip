@@ -137,14 +137,53 @@ struct PairArgs {
     const int4* plist;     // explicit pairs {k2, slot, u, 0}
     PairRec* recs;         // indexed by slot
     EmitRec* erecs;        // indexed by u (second pass)
-    int emit;              // 0: pass 1 (cells), 1: pass 2 (threshold emit)
     int CPP;               // waves per pair (column chunks)
     float coef;            // delta = coef * n1 * n2 + sub * (n1 + n2)
     float sub;
     float tau_t;           // tie margin in t units
+    int stage;             // 1: image k1's rays and the phase table live in LDS
 };
 
-template <typename OpT, int CT, int NK, bool SYM>
+constexpr int WCAP = 64;   // per-wave list of candidate cells (one per lane)
+
+__device__ inline unsigned long long lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Maximum over the 64 lanes with DPP row operations and 4 lane reads.
+__device__ inline float wave_max_dpp(float v) {
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0xb1, 0xf, 0xf, false)));  // quad_perm [1,0,3,2]
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x4e, 0xf, 0xf, false)));  // quad_perm [2,3,0,1]
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x124, 0xf, 0xf, false))); // row_ror:4
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x128, 0xf, 0xf, false))); // row_ror:8
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+// Dynamic LDS bytes of pair_kernel for a launch shape.
+__host__ __device__ inline size_t pair_lds_bytes(int Rp, int osz, int W, int NE, int L, int stage) {
+    size_t b = 64 * (size_t)(Rp * 2 * osz + 16) + 384 + (size_t)W * WCAP * 8;
+    if (stage) b += (size_t)(NE + L) * Rp * sizeof(float2);
+    return b;
+}
+
+// Persistent all-pairs kernel.  A workgroup takes a tile (one image k1 and up
+// to W/CPP partner images k2), stages k1's rays in LDS, and sweeps the rows
+// (e, l1) of k1's shift-expanded rays in A-tiles of 32 rows.  Each A-tile is
+// generated once in LDS (rays x phase, rounded to the operand type) and
+// multiplied by every wave against its k2's columns, which stay in
+// registers.  The epilogue reduces each lane's 32x32 output tile to one
+// "cell" maximum of t = X + |Y|.
+//   EMIT = false (pass 1): each wave keeps every cell that comes within the
+//     certified margin of the pair's running maximum; at the end the cells
+//     within the margin of the pair's final maximum go to the record.
+//   EMIT = true (pass 2): every candidate at or above the pair's threshold T
+//     from pass 1 is listed.
+template <typename OpT, int CT, int NK, bool SYM, bool EMIT>
 __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
     extern __shared__ __align__(16) char smem[];
     using O = Op<OpT>;
@@ -158,15 +197,27 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
     char* abuf0 = smem;
     char* abuf1 = smem + 32 * RS;
     int* s_i = (int*)(smem + 64 * RS);        // [0] tile, [1..] per-pair counters
-    float* s_pm = (float*)(s_i + 32);         // per-wave maxima
-    int* s_unc = (int*)(s_pm + 32);           // per-pair uncertain flags
+    float* s_pm = (float*)(s_i + 32);         // per-wave running maxima
+    int* s_unc = (int*)(s_pm + 32);           // per-pair flags
+    float2* s_list = (float2*)(s_unc + 32);   // [W][WCAP] (value, id bits)
+    float2* s_ph = s_list + W * WCAP;         // [NE][Rp]
+    float2* s_p1 = s_ph + (size_t)g.NE * Rp;  // [L][Rp]
+    float2* wlist = s_list + wave * WCAP;
     const int pi = wave / a.CPP, chunk = wave % a.CPP;
-    const int items = 32 * (Rp / 8);
+    const int hq = Rp / 2;                    // float4 chunks (2 complex) per row
+    const int nchunk = 32 * hq;
+    const float2* ph_src = a.stage ? s_ph : a.ph32;
+
+    if (a.stage) {
+        const int n4 = g.NE * Rp / 2;
+        const float4* src = (const float4*)a.ph32;
+        for (int i = tid; i < n4; i += blockDim.x) ((float4*)s_ph)[i] = src[i];
+    }
 
     for (;;) {
         if (tid == 0) s_i[0] = atomicAdd(a.counter, 1);
         if (tid >= 1 && tid < 32) s_i[tid] = 0;
-        if (tid < 32) s_unc[tid] = 0;
+        if (tid < 32) { s_unc[tid] = 0; s_pm[tid] = -INFINITY; }
         __syncthreads();
         const int t = s_i[0];
         if (t >= a.ntiles) break;
@@ -177,6 +228,12 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         if (active) {
             if (tile.w >= 0) { k2 = tile.y + pi; slot = tile.w + pi; }
             else { int4 pe = a.plist[tile.y + pi]; k2 = pe.x; slot = pe.y; uidx = pe.z; }
+        }
+        const float2* p1_src = a.stage ? s_p1 : a.P32 + (size_t)k1 * g.L * Rp;
+        if (a.stage) {
+            const int n4 = g.L * Rp / 2;
+            const float4* src = (const float4*)(a.P32 + (size_t)k1 * g.L * Rp);
+            for (int i = tid; i < n4; i += blockDim.x) ((float4*)s_p1)[i] = src[i];
         }
         // ---- B operand (image k2, this wave's CT column tiles) in registers
         frag Bre[CT][NK], Bim[CT][NK];
@@ -200,51 +257,50 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
                 }
             }
         }
+        const float n1 = a.nrm[k1], n2 = a.nrm[k2];
+        const float margin = 2.0f * (a.coef * n1 * n2 + a.sub * (n1 + n2)) + a.tau_t;
         float thr = -INFINITY;
-        if (a.emit && active) thr = a.recs[slot].T;
-        float b1 = -INFINITY, b2 = -INFINITY;
-        int i1 = -1;
+        if (EMIT && active) thr = a.recs[slot].T;
+        float b = -INFINITY;   // lane's running cell maximum
+        float pm = -INFINITY;  // pair's running maximum (all waves, previous A-tile)
+        int wc = 0;            // entries in this wave's cell list
+        bool lost = false;     // a cell within the margin could not be kept
 
-        // ---- A-tile generation: rows (e,l1) of image k1 times the shift phase
+        // ---- A-tile generation: rows (e,l1) of image k1 times the shift
+        // phase, two complex values (one float4) per item, consecutive
+        // threads on consecutive 16-byte chunks
         auto gen = [&](int at, char* buf) {
-            for (int it = tid; it < items; it += blockDim.x) {
-                const int row = it / (Rp / 8), kc = it % (Rp / 8);
+            for (int it = tid; it < nchunk; it += blockDim.x) {
+                const int row = it / hq, q = it - row * hq;
                 const int gr = at * 32 + row;
-                float re[8], im[8];
+                float re0 = 0.f, re1 = 0.f, im0 = 0.f, im1 = 0.f;
                 if (gr < g.NROWS) {
                     const int e = gr / g.L, l1 = gr - e * g.L;
-                    const float4* p = (const float4*)(a.P32 + ((size_t)k1 * g.L + l1) * Rp + 8 * kc);
-                    const float4* q = (const float4*)(a.ph32 + (size_t)e * Rp + 8 * kc);
-                    const float sg = (!SYM && e >= g.S) ? -1.0f : 1.0f;
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        float4 pv = p[u], qv = q[u];
-                        re[2 * u] = fmaf(pv.x, qv.x, -pv.y * qv.y);
-                        im[2 * u] = sg * fmaf(pv.x, qv.y, pv.y * qv.x);
-                        re[2 * u + 1] = fmaf(pv.z, qv.z, -pv.w * qv.w);
-                        im[2 * u + 1] = sg * fmaf(pv.z, qv.w, pv.w * qv.z);
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 8; u++) { re[u] = 0.f; im[u] = 0.f; }
+                    const float4 pv = *(const float4*)(p1_src + (size_t)l1 * Rp + 2 * q);
+                    const float4 qv = *(const float4*)(ph_src + (size_t)e * Rp + 2 * q);
+                    re0 = fmaf(pv.x, qv.x, -pv.y * qv.y);
+                    im0 = fmaf(pv.x, qv.y, pv.y * qv.x);
+                    re1 = fmaf(pv.z, qv.z, -pv.w * qv.w);
+                    im1 = fmaf(pv.z, qv.w, pv.w * qv.z);
+                    if (!SYM && e >= g.S) { im0 = -im0; im1 = -im1; }
                 }
-                OpT* dre = (OpT*)(buf + row * RS) + 8 * kc;
+                OpT* dre = (OpT*)(buf + row * RS) + 2 * q;
                 OpT* dim = dre + Rp;
                 if constexpr (O::KSTEP == 16) {
-                    frag vr, vi;
-#pragma unroll
-                    for (int u = 0; u < 8; u++) { vr[u] = (OpT)re[u]; vi[u] = (OpT)im[u]; }
-                    *(frag*)dre = vr;
-                    *(frag*)dim = vi;
+                    typedef OpT op2 __attribute__((ext_vector_type(2)));
+                    op2 vr, vi;
+                    vr[0] = (OpT)re0; vr[1] = (OpT)re1;
+                    vi[0] = (OpT)im0; vi[1] = (OpT)im1;
+                    *(op2*)dre = vr;
+                    *(op2*)dim = vi;
                 } else {
-                    ((float4*)dre)[0] = make_float4(re[0], re[1], re[2], re[3]);
-                    ((float4*)dre)[1] = make_float4(re[4], re[5], re[6], re[7]);
-                    ((float4*)dim)[0] = make_float4(im[0], im[1], im[2], im[3]);
-                    ((float4*)dim)[1] = make_float4(im[4], im[5], im[6], im[7]);
+                    *(float2*)dre = make_float2(re0, re1);
+                    *(float2*)dim = make_float2(im0, im1);
                 }
             }
         };
 
+        if (a.stage) __syncthreads();  // k1's rays staged
         gen(0, abuf0);
         __syncthreads();
         for (int at = 0; at < g.NAT; at++) {
@@ -252,91 +308,143 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
             char* nxt = (at & 1) ? abuf0 : abuf1;
             if (at + 1 < g.NAT) gen(at + 1, nxt);
             if (active) {
-                f32x16 X[CT], Y[CT];
                 const char* arow = cur + lc * RS;
+                float m[CT];
+                f32x16 Xs[O::KSTEP == 16 ? 1 : CT], Ys[O::KSTEP == 16 ? 1 : CT];
+                if constexpr (O::KSTEP != 16) {
+                    // FP32 operands: one k-step at a time over every column tile
 #pragma unroll
-                for (int ks = 0; ks < NK; ks++) {
-                    frag fa, fb;
-                    if constexpr (O::KSTEP == 16) {
-                        fa = *(const frag*)((const OpT*)arow + 16 * ks + 8 * h);
-                        fb = *(const frag*)((const OpT*)arow + Rp + 16 * ks + 8 * h);
-                    } else {
-                        fa = ((const float*)arow)[h * (Rp / 2) + ks];
-                        fb = ((const float*)arow)[Rp + h * (Rp / 2) + ks];
-                    }
+                    for (int ks = 0; ks < NK; ks++) {
+                        const frag fa = ((const float*)arow)[h * (Rp / 2) + ks];
+                        const frag fb = ((const float*)arow)[Rp + h * (Rp / 2) + ks];
 #pragma unroll
-                    for (int ct = 0; ct < CT; ct++) {
-                        if (ks == 0) {
-                            f32x16 z = {0.f};
-                            X[ct] = O::mma(fa, Bre[ct][0], z);
-                            Y[ct] = O::mma(fb, Bim[ct][0], z);
-                        } else {
-                            X[ct] = O::mma(fa, Bre[ct][ks], X[ct]);
-                            Y[ct] = O::mma(fb, Bim[ct][ks], Y[ct]);
+                        for (int ct = 0; ct < CT; ct++) {
+                            if (ks == 0) {
+                                f32x16 z = {0.f};
+                                Xs[ct] = O::mma(fa, Bre[ct][0], z);
+                                Ys[ct] = O::mma(fb, Bim[ct][0], z);
+                            } else {
+                                Xs[ct] = O::mma(fa, Bre[ct][ks], Xs[ct]);
+                                Ys[ct] = O::mma(fb, Bim[ct][ks], Ys[ct]);
+                            }
                         }
                     }
                 }
-                // ---- fused epilogue: best of both maps, cell bookkeeping
+                frag fa[O::KSTEP == 16 ? NK : 1], fb[O::KSTEP == 16 ? NK : 1];
+                if constexpr (O::KSTEP == 16) {
+#pragma unroll
+                    for (int ks = 0; ks < NK; ks++) {
+                        fa[ks] = *(const frag*)((const OpT*)arow + 16 * ks + 8 * h);
+                        fb[ks] = *(const frag*)((const OpT*)arow + Rp + 16 * ks + 8 * h);
+                    }
+                }
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++) {
-                    float m = -INFINITY;
+                    f32x16 X, Y;
+                    if constexpr (O::KSTEP == 16) {
+                        // 16-bit operands: one column tile at a time so its
+                        // epilogue overlaps the next tile's MFMAs
+                        f32x16 z = {0.f};
+                        X = O::mma(fa[0], Bre[ct][0], z);
+                        Y = O::mma(fb[0], Bim[ct][0], z);
 #pragma unroll
-                    for (int v = 0; v < 16; v++) {
-                        const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
-                        m = fmaxf(m, tv);
+                        for (int ks = 1; ks < NK; ks++) {
+                            X = O::mma(fa[ks], Bre[ct][ks], X);
+                            Y = O::mma(fb[ks], Bim[ct][ks], Y);
+                        }
+                    } else {
+                        X = Xs[ct];
+                        Y = Ys[ct];
                     }
-                    if (!a.emit) {
-                        const int cid = at * g.NCT + chunk * CT + ct;
-                        const bool gt = m > b1;
-                        b2 = gt ? b1 : fmaxf(b2, m);
-                        i1 = gt ? cid : i1;
-                        b1 = gt ? m : b1;
-                    } else if (m >= thr) {
-                        const int col = (chunk * CT + ct) * 32 + lc;
+                    // best of both maps over the cell: t = X + |Y|
+                    float mv = -INFINITY;
 #pragma unroll
-                        for (int v = 0; v < 16; v++) {
-                            const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
-                            const int row = at * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-                            if (tv >= thr && row < g.NROWS && col < g.L) {
-                                const int pos = atomicAdd(&s_i[1 + pi], 1);
-                                if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
+                    for (int v = 0; v < 16; v++)
+                        mv = fmaxf(mv, SYM ? X[v] + fabsf(Y[v]) : X[v] + Y[v]);
+                    m[ct] = mv;
+                    if constexpr (EMIT) {
+                        if (mv >= thr) {
+                            const int col = (chunk * CT + ct) * 32 + lc;
+#pragma unroll
+                            for (int v = 0; v < 16; v++) {
+                                const float tv = SYM ? X[v] + fabsf(Y[v]) : X[v] + Y[v];
+                                const int row = at * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+                                if (tv >= thr && row < g.NROWS && col < g.L) {
+                                    const int pos = atomicAdd(&s_i[1 + pi], 1);
+                                    if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
+                                }
                             }
+                        }
+                    }
+                }
+                if constexpr (!EMIT) {
+                    float mm = m[0];
+#pragma unroll
+                    for (int ct = 1; ct < CT; ct++) mm = fmaxf(mm, m[ct]);
+                    b = fmaxf(b, mm);
+                    // wave-uniform threshold from the running maxima seen so
+                    // far (all <= the pair's final maximum)
+                    const float wm = wave_max_dpp(b);
+                    if (lane == 0) s_pm[wave] = wm;
+                    float th = fmaxf(wm, pm) - margin;
+                    if (__ballot(mm >= th && mm > -INFINITY)) {
+                        // rare once the pair's maximum has been seen
+#pragma unroll
+                        for (int ct = 0; ct < CT; ct++) {
+                            bool e = (m[ct] >= th) && (m[ct] > -INFINITY);
+                            unsigned long long mask = __ballot(e);
+                            if (!mask) continue;
+                            int cnt = __popcll(mask);
+                            if (wc + cnt > WCAP) {
+                                // compact: drop entries below the current threshold
+                                const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
+                                const bool keep = lane < wc && ent.x >= th;
+                                const unsigned long long km = __ballot(keep);
+                                __builtin_amdgcn_wave_barrier();
+                                if (keep) wlist[__popcll(km & lanemask_lt())] = ent;
+                                wc = __popcll(km);
+                                __builtin_amdgcn_wave_barrier();
+                            }
+                            if (e) {
+                                const int pos = wc + __popcll(mask & lanemask_lt());
+                                const uint32_t id = ((uint32_t)(at * g.NCT + chunk * CT + ct) << 6) | (uint32_t)lane;
+                                if (pos < WCAP) wlist[pos] = make_float2(m[ct], __uint_as_float(id));
+                            }
+                            if (wc + cnt > WCAP) lost = true;
+                            wc = min(wc + cnt, WCAP);
+                            __builtin_amdgcn_wave_barrier();
                         }
                     }
                 }
             }
             __syncthreads();
+            if (!EMIT && active) {
+                for (int c = 0; c < a.CPP; c++) pm = fmaxf(pm, s_pm[pi * a.CPP + c]);
+            }
         }
 
         // ---- per-pair finalisation
-        if (!a.emit) {
-            const float wm = wave_max(b1);
-            if (lane == 0) s_pm[wave] = wm;
-            __syncthreads();
+        if constexpr (!EMIT) {
+            // s_pm holds every wave's final maximum (written before the last barrier)
             if (active) {
                 float M = -INFINITY;
                 for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
-                const float n1 = a.nrm[k1], n2 = a.nrm[k2];
-                const float delta = a.coef * n1 * n2 + a.sub * (n1 + n2);
-                const float T = M - 2.0f * delta - a.tau_t;
-                const bool em = (i1 >= 0) && (b1 >= T);
-                const bool unc = (b2 >= T);
-                if (em) {
+                const float T = M - margin;
+                const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
+                if (lane < wc && ent.x >= T) {
                     const int pos = atomicAdd(&s_i[1 + pi], 1);
-                    if (pos < MAXC) a.recs[slot].cell[pos] = ((uint32_t)i1 << 6) | (uint32_t)lane;
+                    if (pos < MAXC) a.recs[slot].cell[pos] = __float_as_uint(ent.y);
                 }
-                if (unc) atomicOr(&s_unc[pi], 1);
+                if (lost) atomicOr(&s_unc[pi], 1);
             }
             __syncthreads();
             if (active && chunk == 0 && lane == 0) {
                 float M = -INFINITY;
                 for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
-                const float n1 = a.nrm[k1], n2 = a.nrm[k2];
-                const float delta = a.coef * n1 * n2 + a.sub * (n1 + n2);
                 const int n = s_i[1 + pi];
                 PairRec* r = a.recs + slot;
                 r->M = M;
-                r->T = M - 2.0f * delta - a.tau_t;
+                r->T = M - margin;
                 r->k1 = k1;
                 r->k2 = k2;
                 r->ncells = n < MAXC ? n : MAXC;
@@ -375,6 +483,8 @@ struct RescoreArgs {
     int* fb_count;            // pass 2: overflowing pairs appended here
     int* fb_list;
     unsigned long long* n64;  // FP64 re-scored candidates (stats)
+    unsigned long long* ncell;  // cells re-scored (stats)
+    unsigned long long* nemit;  // second-pass candidates (stats)
     int32_t* cl1; int32_t* cl2; int32_t* jidx; double* corr; double* shift1d;
 };
 
@@ -425,36 +535,6 @@ __device__ inline void cand_rc(const RescoreArgs& a, const PairRec& rec, const E
     if (row >= g.NROWS || col >= g.L) row = -1;
 }
 
-// FP32 value of the (up to) two candidates of row (e,l1) x column l2.
-__device__ inline void eval32(const RescoreArgs& a, int k1, int k2, int row, int col,
-                              float& v1, long long& q1, float& v2, long long& q2) {
-    const Geom& g = a.g;
-    v1 = v2 = -INFINITY;
-    q1 = q2 = 0x7fffffffffffffffLL;
-    if (row < 0) return;
-    const int e = row / g.L, l1 = row - e * g.L;
-    const float2* p = a.P32 + ((size_t)k1 * g.L + l1) * g.Rp;
-    const float2* b = a.P32 + ((size_t)k2 * g.L + col) * g.Rp;
-    const float2* ph = a.ph32 + (size_t)e * g.Rp;
-    float X = 0.f, Y = 0.f;
-    for (int r = 0; r < g.R; r++) {
-        const float2 pv = p[r], qv = ph[r], bv = b[r];
-        const float ar = fmaf(pv.x, qv.x, -pv.y * qv.y);
-        const float ai = fmaf(pv.x, qv.y, pv.y * qv.x);
-        X = fmaf(ar, bv.x, X);
-        Y = fmaf(ai, bv.y, Y);
-    }
-    if (g.sym) {
-        v1 = 2.f * (X + Y); q1 = tie_key(e, col, l1, g.L);
-        v2 = 2.f * (X - Y); q2 = tie_key(g.S - 1 - e, g.L + col, l1, g.L);
-    } else if (e < g.S) {
-        v1 = 2.f * (X + Y); q1 = tie_key(e, col, l1, g.L);
-    } else {
-        // rows e >= S carry the phase of -s_{e-S}: C2(s_{e-S}) = 2(X - Y)
-        v2 = 2.f * (X - Y); q2 = tie_key(e - g.S, g.L + col, l1, g.L);
-    }
-}
-
 __device__ inline double eval64(const RescoreArgs& a, int k1, int k2, long long key) {
     const Geom& g = a.g;
     const int l1 = (int)(key % g.L);
@@ -485,10 +565,20 @@ __device__ inline void write_result(const RescoreArgs& a, int slot, double v, lo
     }
 }
 
-// One wave per pair.
+constexpr int RS_MAXV = 2 * MAXE;  // FP32 values kept per pair (2 per row)
+
+// One wave per pair.  FP32 re-score with 4 lanes per candidate row (16 rows
+// per step, contiguous 16-byte loads), values kept in LDS; then FP64 re-score
+// of every candidate within tau of the FP32 best, one lane per candidate with
+// the oracle's sequential arithmetic.
 __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
+    __shared__ float s_val[4][RS_MAXV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = blockDim.x >> 6;
+    float* sval = s_val[wave];
+    const Geom& g = a.g;
+    const int q = lane & 3, rl = lane >> 2;  // k-quarter, row within the step
+    const int kq = g.Rp / 4;                 // multiple of 4
     for (int item = blockIdx.x * W + wave; item < a.n_items; item += gridDim.x * W) {
         const int slot = a.mode == 0 ? item : a.ulist[item];
         const PairRec rec = a.recs[slot];
@@ -505,23 +595,63 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
                 continue;
             }
             nrows = 16 * rec.ncells;
+            if (lane == 0 && a.ncell) atomicAdd(a.ncell, (unsigned long long)rec.ncells);
         } else {
             if (er->overflow) {
                 if (lane == 0) { const int u = atomicAdd(a.fb_count, 1); a.fb_list[u] = slot; }
                 continue;
             }
             nrows = er->n;
+            if (lane == 0 && a.nemit) atomicAdd(a.nemit, (unsigned long long)nrows);
         }
-        // ---- FP32 re-score, one lane per (row, col)
+        // ---- FP32 re-score
         float bv = -INFINITY;
         long long bk = 0x7fffffffffffffffLL;
-        for (int base = 0; base < nrows; base += 64) {
-            const int q = base + lane;
+        for (int base = 0; base < nrows; base += 16) {
+            const int r = base + rl;
             int row = -1, col = 0;
-            if (q < nrows) cand_rc(a, rec, er, q, row, col);
-            float v1, v2;
-            long long q1, q2;
-            eval32(a, k1, k2, row, col, v1, q1, v2, q2);
+            if (r < nrows) cand_rc(a, rec, er, r, row, col);
+            float X = 0.f, Y = 0.f;
+            int e = 0, l1 = 0;
+            if (row >= 0) {
+                e = row / g.L;
+                l1 = row - e * g.L;
+                const float4* p = (const float4*)(a.P32 + ((size_t)k1 * g.L + l1) * g.Rp + q * kq);
+                const float4* bb = (const float4*)(a.P32 + ((size_t)k2 * g.L + col) * g.Rp + q * kq);
+                const float4* ph = (const float4*)(a.ph32 + (size_t)e * g.Rp + q * kq);
+                for (int i = 0; i < kq / 2; i++) {
+                    const float4 pv = p[i], qv = ph[i], b4 = bb[i];
+                    const float ar0 = fmaf(pv.x, qv.x, -pv.y * qv.y);
+                    const float ai0 = fmaf(pv.x, qv.y, pv.y * qv.x);
+                    const float ar1 = fmaf(pv.z, qv.z, -pv.w * qv.w);
+                    const float ai1 = fmaf(pv.z, qv.w, pv.w * qv.z);
+                    X = fmaf(ar0, b4.x, X);
+                    Y = fmaf(ai0, b4.y, Y);
+                    X = fmaf(ar1, b4.z, X);
+                    Y = fmaf(ai1, b4.w, Y);
+                }
+            }
+            X += __shfl_xor(X, 1, 64);
+            Y += __shfl_xor(Y, 1, 64);
+            X += __shfl_xor(X, 2, 64);
+            Y += __shfl_xor(Y, 2, 64);
+            float v1 = -INFINITY, v2 = -INFINITY;
+            long long q1 = 0x7fffffffffffffffLL, q2 = 0x7fffffffffffffffLL;
+            if (row >= 0) {
+                if (g.sym) {
+                    v1 = 2.f * (X + Y); q1 = tie_key(e, col, l1, g.L);
+                    v2 = 2.f * (X - Y); q2 = tie_key(g.S - 1 - e, g.L + col, l1, g.L);
+                } else if (e < g.S) {
+                    v1 = 2.f * (X + Y); q1 = tie_key(e, col, l1, g.L);
+                } else {
+                    // rows e >= S carry the phase of -s_{e-S}: C2(s_{e-S}) = 2(X - Y)
+                    v2 = 2.f * (X - Y); q2 = tie_key(e - g.S, g.L + col, l1, g.L);
+                }
+            }
+            if (q == 0 && r < nrows && 2 * r + 1 < RS_MAXV) {
+                sval[2 * r] = v1;
+                sval[2 * r + 1] = v2;
+            }
             if (better(v1, q1, bv, bk)) { bv = v1; bk = q1; }
             if (better(v2, q2, bv, bk)) { bv = v2; bk = q2; }
         }
@@ -534,27 +664,25 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         double fv = (double)bv;
         long long fk = bk;
         if (a.fp64 && bv > -INFINITY) {
+            __builtin_amdgcn_wave_barrier();
             // ---- FP64 re-score of every candidate within tau of the FP32 best
             const float lim = bv - a.tau;
             double b64 = -INFINITY;
             long long k64 = 0x7fffffffffffffffLL;
             int n64 = 0;
-            for (int base = 0; base < nrows; base += 64) {
-                const int q = base + lane;
-                int row = -1, col = 0;
-                if (q < nrows) cand_rc(a, rec, er, q, row, col);
-                float v1, v2;
-                long long q1, q2;
-                eval32(a, k1, k2, row, col, v1, q1, v2, q2);
-                if (v1 >= lim) {
-                    const double v = eval64(a, k1, k2, q1);
+            const int nv = min(2 * nrows, RS_MAXV);
+            for (int base = 0; base < nv; base += 64) {
+                const int iv = base + lane;
+                if (iv < nv && sval[iv] >= lim) {
+                    int row, col;
+                    cand_rc(a, rec, er, iv >> 1, row, col);
+                    const int e = row / g.L, l1 = row - e * g.L;
+                    long long key;
+                    if (g.sym) key = (iv & 1) ? tie_key(g.S - 1 - e, g.L + col, l1, g.L) : tie_key(e, col, l1, g.L);
+                    else key = e < g.S ? tie_key(e, col, l1, g.L) : tie_key(e - g.S, g.L + col, l1, g.L);
+                    const double v = eval64(a, k1, k2, key);
                     n64++;
-                    if (!isnan(v) && better64(v, q1, b64, k64)) { b64 = v; k64 = q1; }
-                }
-                if (v2 >= lim) {
-                    const double v = eval64(a, k1, k2, q2);
-                    n64++;
-                    if (!isnan(v) && better64(v, q2, b64, k64)) { b64 = v; k64 = q2; }
+                    if (!isnan(v) && better64(v, key, b64, k64)) { b64 = v; k64 = key; }
                 }
             }
 #pragma unroll
@@ -569,6 +697,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             fk = k64;
         }
         if (lane == 0) write_result(a, slot, fv, fk);
+        __builtin_amdgcn_wave_barrier();
     }
 }
 

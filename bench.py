@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy"],
+                    help="pairs: one BF16 call on the workload and nothing else (for counter "
+                         "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -52,6 +55,15 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.mode == "copy":
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
+        return
 
     from aspire_amd import sim
     from aspire_amd.clmatrix import clmatrix_records
@@ -71,6 +83,12 @@ def main():
 
     def run(prec, x, variant=0):
         return clmatrix_records(x, ms, step, precision=prec, variant=variant)
+
+    if args.mode == "pairs":
+        rec, stats = run(args.precision, pf, args.variant)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "pairs", "K": K, "stats": stats}))
+        return
 
     for i in range(args.warmup):
         rec, stats = run(args.precision, pf, args.variant)
