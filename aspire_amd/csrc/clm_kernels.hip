@@ -192,8 +192,8 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = blockDim.x >> 6;
     const int h = lane >> 5, lc = lane & 31;
-    const int Rp = g.Rp;
-    const int RS = Rp * 2 * (int)sizeof(OpT) + 16;  // LDS row stride (bytes)
+    constexpr int Rp = NK * O::KSTEP;                   // padded ray length (== g.Rp)
+    constexpr int RS = Rp * 2 * (int)sizeof(OpT) + 16;  // LDS row stride (bytes)
     char* abuf0 = smem;
     char* abuf1 = smem + 32 * RS;
     int* s_i = (int*)(smem + 64 * RS);        // [0] tile, [1..] per-pair counters
@@ -204,8 +204,8 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
     float2* s_p1 = s_ph + (size_t)g.NE * Rp;  // [L][Rp]
     float2* wlist = s_list + wave * WCAP;
     const int pi = wave / a.CPP, chunk = wave % a.CPP;
-    const int hq = Rp / 2;                    // float4 chunks (2 complex) per row
-    const int nchunk = 32 * hq;
+    constexpr int hq = Rp / 2;                // float4 chunks (2 complex) per row
+    constexpr int nchunk = 32 * hq;
     const float2* ph_src = a.stage ? s_ph : a.ph32;
 
     if (a.stage) {
@@ -268,23 +268,28 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
 
         // ---- A-tile generation: rows (e,l1) of image k1 times the shift
         // phase, two complex values (one float4) per item, consecutive
-        // threads on consecutive 16-byte chunks
+        // threads on consecutive 16-byte chunks.  e = row / L by a float
+        // reciprocal with one correction step (rows < 2^24).
+        const float invL = 1.0f / (float)g.L;
         auto gen = [&](int at, char* buf) {
             for (int it = tid; it < nchunk; it += blockDim.x) {
-                const int row = it / hq, q = it - row * hq;
+                const int row = it / hq, qq = it - row * hq;
                 const int gr = at * 32 + row;
                 float re0 = 0.f, re1 = 0.f, im0 = 0.f, im1 = 0.f;
                 if (gr < g.NROWS) {
-                    const int e = gr / g.L, l1 = gr - e * g.L;
-                    const float4 pv = *(const float4*)(p1_src + (size_t)l1 * Rp + 2 * q);
-                    const float4 qv = *(const float4*)(ph_src + (size_t)e * Rp + 2 * q);
+                    int e = (int)((float)gr * invL);
+                    if (e * g.L > gr) e--;
+                    if ((e + 1) * g.L <= gr) e++;
+                    const int l1 = gr - e * g.L;
+                    const float4 pv = *(const float4*)(p1_src + (size_t)l1 * Rp + 2 * qq);
+                    const float4 qv = *(const float4*)(ph_src + (size_t)e * Rp + 2 * qq);
                     re0 = fmaf(pv.x, qv.x, -pv.y * qv.y);
                     im0 = fmaf(pv.x, qv.y, pv.y * qv.x);
                     re1 = fmaf(pv.z, qv.z, -pv.w * qv.w);
                     im1 = fmaf(pv.z, qv.w, pv.w * qv.z);
                     if (!SYM && e >= g.S) { im0 = -im0; im1 = -im1; }
                 }
-                OpT* dre = (OpT*)(buf + row * RS) + 2 * q;
+                OpT* dre = (OpT*)(buf + row * RS) + 2 * qq;
                 OpT* dim = dre + Rp;
                 if constexpr (O::KSTEP == 16) {
                     typedef OpT op2 __attribute__((ext_vector_type(2)));
@@ -306,77 +311,87 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         for (int at = 0; at < g.NAT; at++) {
             char* cur = (at & 1) ? abuf1 : abuf0;
             char* nxt = (at & 1) ? abuf0 : abuf1;
-            if (at + 1 < g.NAT) gen(at + 1, nxt);
-            if (active) {
-                const char* arow = cur + lc * RS;
-                float m[CT];
-                f32x16 Xs[O::KSTEP == 16 ? 1 : CT], Ys[O::KSTEP == 16 ? 1 : CT];
-                if constexpr (O::KSTEP != 16) {
-                    // FP32 operands: one k-step at a time over every column tile
+            const char* arow = cur + lc * RS;
+            float m[CT];
+            f32x16 X[CT], Y[CT];
+            frag fa[O::KSTEP == 16 ? NK : 1], fb[O::KSTEP == 16 ? NK : 1];
+            // best of both maps over a cell: t = X + |Y|; pass 2 also lists
+            // every candidate at or above the threshold
+            auto epi = [&](int ct) {
+                float mv = -INFINITY;
 #pragma unroll
-                    for (int ks = 0; ks < NK; ks++) {
-                        const frag fa = ((const float*)arow)[h * (Rp / 2) + ks];
-                        const frag fb = ((const float*)arow)[Rp + h * (Rp / 2) + ks];
+                for (int v = 0; v < 16; v++)
+                    mv = fmaxf(mv, SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v]);
+                m[ct] = mv;
+                if constexpr (EMIT) {
+                    if (mv >= thr) {
+                        const int col = (chunk * CT + ct) * 32 + lc;
 #pragma unroll
-                        for (int ct = 0; ct < CT; ct++) {
-                            if (ks == 0) {
-                                f32x16 z = {0.f};
-                                Xs[ct] = O::mma(fa, Bre[ct][0], z);
-                                Ys[ct] = O::mma(fb, Bim[ct][0], z);
-                            } else {
-                                Xs[ct] = O::mma(fa, Bre[ct][ks], Xs[ct]);
-                                Ys[ct] = O::mma(fb, Bim[ct][ks], Ys[ct]);
+                        for (int v = 0; v < 16; v++) {
+                            const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
+                            const int row = at * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+                            if (tv >= thr && row < g.NROWS && col < g.L) {
+                                const int pos = atomicAdd(&s_i[1 + pi], 1);
+                                if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
                             }
                         }
                     }
                 }
-                frag fa[O::KSTEP == 16 ? NK : 1], fb[O::KSTEP == 16 ? NK : 1];
-                if constexpr (O::KSTEP == 16) {
+            };
+            auto mma16 = [&](int ct) {
+                f32x16 z = {0.f};
+                X[ct] = O::mma(fa[0], Bre[ct][0], z);
+                Y[ct] = O::mma(fb[0], Bim[ct][0], z);
+#pragma unroll
+                for (int ks = 1; ks < NK; ks++) {
+                    X[ct] = O::mma(fa[ks], Bre[ct][ks], X[ct]);
+                    Y[ct] = O::mma(fb[ks], Bim[ct][ks], Y[ct]);
+                }
+            };
+            if constexpr (O::KSTEP == 16) {
+                // A fragments of this tile, then the first column tile's MFMAs,
+                // then the next tile's generation while they run
+                if (active) {
 #pragma unroll
                     for (int ks = 0; ks < NK; ks++) {
                         fa[ks] = *(const frag*)((const OpT*)arow + 16 * ks + 8 * h);
                         fb[ks] = *(const frag*)((const OpT*)arow + Rp + 16 * ks + 8 * h);
                     }
+                    mma16(0);
                 }
+                if (at + 1 < g.NAT) gen(at + 1, nxt);
+                if (active) {
 #pragma unroll
-                for (int ct = 0; ct < CT; ct++) {
-                    f32x16 X, Y;
-                    if constexpr (O::KSTEP == 16) {
-                        // 16-bit operands: one column tile at a time so its
-                        // epilogue overlaps the next tile's MFMAs
-                        f32x16 z = {0.f};
-                        X = O::mma(fa[0], Bre[ct][0], z);
-                        Y = O::mma(fb[0], Bim[ct][0], z);
-#pragma unroll
-                        for (int ks = 1; ks < NK; ks++) {
-                            X = O::mma(fa[ks], Bre[ct][ks], X);
-                            Y = O::mma(fb[ks], Bim[ct][ks], Y);
-                        }
-                    } else {
-                        X = Xs[ct];
-                        Y = Ys[ct];
+                    for (int ct = 0; ct < CT; ct++) {
+                        if (ct + 1 < CT) mma16(ct + 1);
+                        epi(ct);
                     }
-                    // best of both maps over the cell: t = X + |Y|
-                    float mv = -INFINITY;
+                }
+            } else {
+                if (at + 1 < g.NAT) gen(at + 1, nxt);
+                if (active) {
+                    // FP32 operands: one k-step at a time over every column tile
 #pragma unroll
-                    for (int v = 0; v < 16; v++)
-                        mv = fmaxf(mv, SYM ? X[v] + fabsf(Y[v]) : X[v] + Y[v]);
-                    m[ct] = mv;
-                    if constexpr (EMIT) {
-                        if (mv >= thr) {
-                            const int col = (chunk * CT + ct) * 32 + lc;
+                    for (int ks = 0; ks < NK; ks++) {
+                        const frag a0 = ((const float*)arow)[h * (Rp / 2) + ks];
+                        const frag b0 = ((const float*)arow)[Rp + h * (Rp / 2) + ks];
 #pragma unroll
-                            for (int v = 0; v < 16; v++) {
-                                const float tv = SYM ? X[v] + fabsf(Y[v]) : X[v] + Y[v];
-                                const int row = at * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-                                if (tv >= thr && row < g.NROWS && col < g.L) {
-                                    const int pos = atomicAdd(&s_i[1 + pi], 1);
-                                    if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
-                                }
+                        for (int ct = 0; ct < CT; ct++) {
+                            if (ks == 0) {
+                                f32x16 z = {0.f};
+                                X[ct] = O::mma(a0, Bre[ct][0], z);
+                                Y[ct] = O::mma(b0, Bim[ct][0], z);
+                            } else {
+                                X[ct] = O::mma(a0, Bre[ct][ks], X[ct]);
+                                Y[ct] = O::mma(b0, Bim[ct][ks], Y[ct]);
                             }
                         }
                     }
+#pragma unroll
+                    for (int ct = 0; ct < CT; ct++) epi(ct);
                 }
+            }
+            if (active) {
                 if constexpr (!EMIT) {
                     float mm = m[0];
 #pragma unroll
