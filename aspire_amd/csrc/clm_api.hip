@@ -245,7 +245,7 @@ int setup(clm_ctx* c, const clm_params* p, Setup& s) {
             s.ph64[(size_t)j * R + r] = make_double2(std::cos(th), std::sin(th));
         }
     if (g.Rp > 0) {
-        s.ph32.assign((size_t)g.NE * g.Rp, make_float2(0.f, 0.f));
+        s.ph32.assign((size_t)(g.NE + 1) * g.Rp, make_float2(0.f, 0.f));  // row NE stays zero
         for (int e = 0; e < g.NE; e++) {
             const double sh = e < g.S ? s.shifts[e] : -s.shifts[e - g.S];
             for (int r = 0; r < R; r++) {

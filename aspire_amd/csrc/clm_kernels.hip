@@ -129,7 +129,7 @@ struct PairArgs {
     Geom g;
     const float2* P32;     // [K][L][Rp]
     const void* Bop;       // [K][2][Lp][Rp] operand planes
-    const float2* ph32;    // [NE][Rp] (cos, sin) of the evaluation shifts
+    const float2* ph32;    // [NE + 1][Rp] (cos, sin) of the evaluation shifts; row NE zero
     const float* nrm;      // [K]
     const int4* tiles;     // {k1, k2_first | list offset, count, slot | -1}
     int ntiles;
@@ -167,7 +167,7 @@ __device__ inline float wave_max_dpp(float v) {
 // Dynamic LDS bytes of pair_kernel for a launch shape.
 __host__ __device__ inline size_t pair_lds_bytes(int Rp, int osz, int W, int NE, int L, int stage) {
     size_t b = 64 * (size_t)(Rp * 2 * osz + 16) + 384 + (size_t)W * WCAP * 8;
-    if (stage) b += (size_t)(NE + L) * Rp * sizeof(float2);
+    if (stage) b += (size_t)(NE + 1 + L) * Rp * sizeof(float2);
     return b;
 }
 
@@ -200,16 +200,13 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
     float* s_pm = (float*)(s_i + 32);         // per-wave running maxima
     int* s_unc = (int*)(s_pm + 32);           // per-pair flags
     float2* s_list = (float2*)(s_unc + 32);   // [W][WCAP] (value, id bits)
-    float2* s_ph = s_list + W * WCAP;         // [NE][Rp]
-    float2* s_p1 = s_ph + (size_t)g.NE * Rp;  // [L][Rp]
+    float2* s_ph = s_list + W * WCAP;         // [NE + 1][Rp], row NE zero
+    float2* s_p1 = s_ph + (size_t)(g.NE + 1) * Rp;  // [L][Rp]
     float2* wlist = s_list + wave * WCAP;
     const int pi = wave / a.CPP, chunk = wave % a.CPP;
-    constexpr int hq = Rp / 2;                // float4 chunks (2 complex) per row
-    constexpr int nchunk = 32 * hq;
-    const float2* ph_src = a.stage ? s_ph : a.ph32;
 
     if (a.stage) {
-        const int n4 = g.NE * Rp / 2;
+        const int n4 = (g.NE + 1) * Rp / 2;
         const float4* src = (const float4*)a.ph32;
         for (int i = tid; i < n4; i += blockDim.x) ((float4*)s_ph)[i] = src[i];
     }
@@ -229,7 +226,6 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
             if (tile.w >= 0) { k2 = tile.y + pi; slot = tile.w + pi; }
             else { int4 pe = a.plist[tile.y + pi]; k2 = pe.x; slot = pe.y; uidx = pe.z; }
         }
-        const float2* p1_src = a.stage ? s_p1 : a.P32 + (size_t)k1 * g.L * Rp;
         if (a.stage) {
             const int n4 = g.L * Rp / 2;
             const float4* src = (const float4*)(a.P32 + (size_t)k1 * g.L * Rp);
@@ -267,42 +263,66 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         bool lost = false;     // a cell within the margin could not be kept
 
         // ---- A-tile generation: rows (e,l1) of image k1 times the shift
-        // phase, two complex values (one float4) per item, consecutive
-        // threads on consecutive 16-byte chunks.  e = row / L by a float
-        // reciprocal with one correction step (rows < 2^24).
-        const float invL = 1.0f / (float)g.L;
+        // phase, rounded once to the operand type.  One item = 4 complex
+        // values of one row (two float4 of rays, two of phases); consecutive
+        // threads take consecutive items.  (g_e0, g_l10) is the (e, l1) of
+        // row 0 of the next tile to generate, advanced by 32 rows per tile.
+        constexpr int QC = Rp / 4;        // items per row
+        constexpr int NC = 32 * QC;       // items per tile
+        int g_e0 = 0, g_l10 = 0;
+        const float2* p1_glb = a.P32 + (size_t)k1 * g.L * Rp;
         auto gen = [&](int at, char* buf) {
-            for (int it = tid; it < nchunk; it += blockDim.x) {
-                const int row = it / hq, qq = it - row * hq;
-                const int gr = at * 32 + row;
-                float re0 = 0.f, re1 = 0.f, im0 = 0.f, im1 = 0.f;
-                if (gr < g.NROWS) {
-                    int e = (int)((float)gr * invL);
-                    if (e * g.L > gr) e--;
-                    if ((e + 1) * g.L <= gr) e++;
-                    const int l1 = gr - e * g.L;
-                    const float4 pv = *(const float4*)(p1_src + (size_t)l1 * Rp + 2 * qq);
-                    const float4 qv = *(const float4*)(ph_src + (size_t)e * Rp + 2 * qq);
-                    re0 = fmaf(pv.x, qv.x, -pv.y * qv.y);
-                    im0 = fmaf(pv.x, qv.y, pv.y * qv.x);
-                    re1 = fmaf(pv.z, qv.z, -pv.w * qv.w);
-                    im1 = fmaf(pv.z, qv.w, pv.w * qv.z);
-                    if (!SYM && e >= g.S) { im0 = -im0; im1 = -im1; }
+            for (int it = tid; it < NC; it += blockDim.x) {
+                const int row = it / QC, q = it - row * QC;
+                int e = g_e0, l1 = g_l10 + row;
+                if (g.L >= 32) {
+                    if (l1 >= g.L) { l1 -= g.L; e++; }
+                } else {
+                    e += l1 / g.L;
+                    l1 -= (l1 / g.L) * g.L;
                 }
-                OpT* dre = (OpT*)(buf + row * RS) + 2 * qq;
+                // rows past the last shift read the zero phase row NE, so
+                // padding rows of the last tile come out as exact zeros
+                const int ep = min(e, g.NE);
+                float4 p0, p1v, q0, q1;
+                if (a.stage) {
+                    const float4* pp = (const float4*)(s_p1 + l1 * Rp + 4 * q);
+                    const float4* qp = (const float4*)(s_ph + ep * Rp + 4 * q);
+                    p0 = pp[0]; p1v = pp[1]; q0 = qp[0]; q1 = qp[1];
+                } else {
+                    const float4* pp = (const float4*)(p1_glb + (size_t)l1 * Rp + 4 * q);
+                    const float4* qp = (const float4*)(a.ph32 + (size_t)ep * Rp + 4 * q);
+                    p0 = pp[0]; p1v = pp[1]; q0 = qp[0]; q1 = qp[1];
+                }
+                float re[4], im[4];
+                re[0] = fmaf(p0.x, q0.x, -p0.y * q0.y);
+                im[0] = fmaf(p0.x, q0.y, p0.y * q0.x);
+                re[1] = fmaf(p0.z, q0.z, -p0.w * q0.w);
+                im[1] = fmaf(p0.z, q0.w, p0.w * q0.z);
+                re[2] = fmaf(p1v.x, q1.x, -p1v.y * q1.y);
+                im[2] = fmaf(p1v.x, q1.y, p1v.y * q1.x);
+                re[3] = fmaf(p1v.z, q1.z, -p1v.w * q1.w);
+                im[3] = fmaf(p1v.z, q1.w, p1v.w * q1.z);
+                if (!SYM && e >= g.S) {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) im[u] = -im[u];
+                }
+                OpT* dre = (OpT*)(buf + row * RS) + 4 * q;
                 OpT* dim = dre + Rp;
                 if constexpr (O::KSTEP == 16) {
-                    typedef OpT op2 __attribute__((ext_vector_type(2)));
-                    op2 vr, vi;
-                    vr[0] = (OpT)re0; vr[1] = (OpT)re1;
-                    vi[0] = (OpT)im0; vi[1] = (OpT)im1;
-                    *(op2*)dre = vr;
-                    *(op2*)dim = vi;
+                    typedef OpT op4 __attribute__((ext_vector_type(4)));
+                    op4 vr, vi;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) { vr[u] = (OpT)re[u]; vi[u] = (OpT)im[u]; }
+                    *(op4*)dre = vr;
+                    *(op4*)dim = vi;
                 } else {
-                    *(float2*)dre = make_float2(re0, re1);
-                    *(float2*)dim = make_float2(im0, im1);
+                    *(float4*)dre = make_float4(re[0], re[1], re[2], re[3]);
+                    *(float4*)dim = make_float4(im[0], im[1], im[2], im[3]);
                 }
             }
+            g_l10 += 32;
+            while (g_l10 >= g.L) { g_l10 -= g.L; g_e0++; }
         };
 
         if (a.stage) __syncthreads();  // k1's rays staged
