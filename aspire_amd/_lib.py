@@ -25,8 +25,10 @@ PRECISIONS = {
 EXPORTS = (
     "clm_version", "clm_default_params", "clm_num_pairs", "clm_create",
     "clm_destroy", "clm_last_error", "clm_clmatrix", "clm_clmatrix_device",
-    "clm_assemble", "clm_shift_equations_csc",
+    "clm_clmatrix_grouped", "clm_autotune", "clm_assemble", "clm_shift_equations_csc",
 )
+
+CLM_GROUP_GRAPH = 1
 
 
 class ClmParams(ctypes.Structure):
@@ -42,6 +44,26 @@ class ClmParams(ctypes.Structure):
         ("tie_margin", ctypes.c_double),
         ("variant", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+    ]
+
+
+class ClmStackIO(ctypes.Structure):
+    _fields_ = [
+        ("pf", ctypes.c_void_p),
+        ("cl1", ctypes.c_void_p),
+        ("cl2", ctypes.c_void_p),
+        ("jidx", ctypes.c_void_p),
+        ("corr", ctypes.c_void_p),
+        ("shift1d", ctypes.c_void_p),
+    ]
+
+
+class ClmTuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("variant", ctypes.c_int32),
+        ("lag", ctypes.c_int32),
+        ("ms", ctypes.c_double),
+        ("pairs", ctypes.c_int64),
     ]
 
 
@@ -99,6 +121,14 @@ def load():
     L.clm_clmatrix_device.restype = ctypes.c_int
     L.clm_clmatrix_device.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(ClmStats)]
+    L.clm_clmatrix_grouped.restype = ctypes.c_int
+    L.clm_clmatrix_grouped.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ClmParams),
+                                       ctypes.POINTER(ClmStackIO), ctypes.c_int32,
+                                       ctypes.POINTER(ClmStats)]
+    L.clm_autotune.restype = ctypes.c_int
+    L.clm_autotune.argtypes = [vp, ctypes.POINTER(ClmParams), vp, ctypes.c_int32,
+                               ctypes.POINTER(ClmTuneEntry), ctypes.c_int32,
+                               ctypes.POINTER(ctypes.c_int32)]
     L.clm_assemble.restype = ctypes.c_int
     L.clm_assemble.argtypes = [ctypes.c_int32, vp, vp, vp, vp, vp, vp]
     L.clm_shift_equations_csc.restype = ctypes.c_int

@@ -31,7 +31,8 @@ def _n_shifts(max_shift, shift_step):
     return int(np.ceil(2 * max_shift / shift_step + 1))
 
 
-def _params(n_r, n_theta, K, dtype, max_shift, shift_step, precision, verbose, tie_margin, variant):
+def _params(n_r, n_theta, K, dtype, max_shift, shift_step, precision, verbose, tie_margin, variant,
+            flags=0):
     p = _lib.default_params()
     p.n_r, p.n_theta, p.K = int(n_r), int(n_theta), int(K)
     p.pf_dtype = dtype
@@ -41,6 +42,7 @@ def _params(n_r, n_theta, K, dtype, max_shift, shift_step, precision, verbose, t
     p.verbose = int(verbose)
     p.tie_margin = float(tie_margin)
     p.variant = int(variant)
+    p.reserved = int(flags)
     return p
 
 
@@ -57,7 +59,7 @@ def _as_matlab_pf(pf: np.ndarray):
 
 
 def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=0,
-                     tie_margin=1e-3, variant=0, verbose=0):
+                     tie_margin=1e-3, variant=0, verbose=0, flags=0):
     """Per-pair records of the common-lines search.
 
     ``pf`` is a numpy array (host path) or a torch tensor on a HIP device of
@@ -73,18 +75,7 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
     if type(pf).__module__.startswith("torch"):
         import torch
 
-        if pf.dim() == 2:
-            pf = pf.unsqueeze(2)
-        n_r, n_theta, K = pf.shape
-        base = pf.permute(2, 1, 0)
-        if not base.is_contiguous():
-            base = base.contiguous()
-        if base.dtype == torch.complex64:
-            dt = _lib.CLM_C64
-        elif base.dtype == torch.complex128:
-            dt = _lib.CLM_C128
-        else:
-            raise TypeError("pf must be complex64 or complex128")
+        base, dt, n_r, n_theta, K = _device_view(pf)
         P = max(K * (K - 1) // 2, 0)
         dev = base.device
         out = dict(cl1=torch.zeros(P, dtype=torch.int32, device=dev),
@@ -92,7 +83,8 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
                    j=torch.zeros(P, dtype=torch.int32, device=dev),
                    corr=torch.zeros(P, dtype=torch.float64, device=dev),
                    shift=torch.zeros(P, dtype=torch.float64, device=dev))
-        p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant)
+        p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant,
+                    flags)
         # torch's own queue must be done with pf before libclm's stream reads it
         torch.cuda.current_stream(dev).synchronize()
         rc = _lib.load().clm_clmatrix_device(
@@ -107,7 +99,8 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
     P = max(K * (K - 1) // 2, 0)
     out = dict(cl1=np.zeros(P, np.int32), cl2=np.zeros(P, np.int32), j=np.zeros(P, np.int32),
                corr=np.zeros(P, np.float64), shift=np.zeros(P, np.float64))
-    p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant)
+    p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant,
+                flags)
     rc = _lib.load().clm_clmatrix(
         ctx.handle, ctypes.byref(p), pfm.ctypes.data_as(ctypes.c_void_p),
         out["cl1"].ctypes.data_as(ctypes.c_void_p), out["cl2"].ctypes.data_as(ctypes.c_void_p),
@@ -115,6 +108,93 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
         out["shift"].ctypes.data_as(ctypes.c_void_p), ctypes.byref(stats))
     ctx.check(rc)
     return out, stats.as_dict()
+
+
+def _device_view(pf):
+    """(base tensor K x n_theta x n_r, dtype code, n_r, n_theta, K) of a
+    column-major n_r x n_theta x K device tensor."""
+    import torch
+
+    if pf.dim() == 2:
+        pf = pf.unsqueeze(2)
+    n_r, n_theta, K = pf.shape
+    base = pf.permute(2, 1, 0)
+    if not base.is_contiguous():
+        base = base.contiguous()
+    if base.dtype == torch.complex64:
+        dt = _lib.CLM_C64
+    elif base.dtype == torch.complex128:
+        dt = _lib.CLM_C128
+    else:
+        raise TypeError("pf must be complex64 or complex128")
+    return base, dt, n_r, n_theta, K
+
+
+def clmatrix_grouped(pfs, max_shift=15, shift_step=1.0, precision="bf16", device=0, tie_margin=1e-3,
+                     variant=0, flags=0, graph=True, out=None):
+    """Several stacks in one device job (libclm's clm_clmatrix_grouped).
+
+    ``pfs`` is a list of device tensors (n_r x n_theta x K_i, column-major
+    memory as for :func:`clmatrix_records`); ``max_shift``, ``shift_step``
+    and ``precision`` are scalars or per-stack lists.  With ``graph`` the job
+    is captured in a HIP graph and replayed on later calls with the same
+    shapes and buffers: pass the previous call's ``out`` to reuse them.
+    Returns ``(records, stats)`` with one record dict per stack.
+    """
+    import torch
+
+    n = len(pfs)
+    per = lambda v: list(v) if isinstance(v, (list, tuple)) else [v] * n
+    ms, ss, pr = per(max_shift), per(shift_step), per(precision)
+    ctx = _lib.context(device)
+    params = (_lib.ClmParams * max(n, 1))()
+    ios = (_lib.ClmStackIO * max(n, 1))()
+    bases, outs = [], []
+    for i, pf in enumerate(pfs):
+        base, dt, n_r, n_theta, K = _device_view(pf)
+        bases.append(base)
+        P = max(K * (K - 1) // 2, 0)
+        dev = base.device
+        o = out[i] if out is not None else dict(
+            cl1=torch.zeros(P, dtype=torch.int32, device=dev),
+            cl2=torch.zeros(P, dtype=torch.int32, device=dev),
+            j=torch.zeros(P, dtype=torch.int32, device=dev),
+            corr=torch.zeros(P, dtype=torch.float64, device=dev),
+            shift=torch.zeros(P, dtype=torch.float64, device=dev))
+        outs.append(o)
+        params[i] = _params(n_r, n_theta, K, dt, ms[i], ss[i], pr[i], 0, tie_margin, variant, flags)
+        ios[i] = _lib.ClmStackIO(base.data_ptr(), o["cl1"].data_ptr(), o["cl2"].data_ptr(),
+                                 o["j"].data_ptr(), o["corr"].data_ptr(), o["shift"].data_ptr())
+    stats = _lib.ClmStats()
+    torch.cuda.current_stream(torch.device("cuda", device)).synchronize()
+    rc = _lib.load().clm_clmatrix_grouped(ctx.handle, n, params, ios,
+                                          _lib.CLM_GROUP_GRAPH if graph else 0, ctypes.byref(stats))
+    ctx.check(rc)
+    return outs, stats.as_dict()
+
+
+def autotune(pf, max_shift=15, shift_step=1.0, precision="bf16", device=0, sample_pairs=200000, flags=0):
+    """Time every pair-kernel variant for this shape on the first
+    ``sample_pairs`` pairs of ``pf`` (device tensor, column-major as for
+    :func:`clmatrix_records`); later calls with ``variant=0`` on this device
+    use the fastest.  Returns one dict per variant (variant = CT*100 + waves
+    per workgroup, lag, ms, pairs, us_per_kpair) and the chosen one."""
+    import torch
+
+    base, dt, n_r, n_theta, K = _device_view(pf)
+    ctx = _lib.context(device)
+    p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, 0, 1e-3, 0, flags)
+    cap = 64
+    ent = (_lib.ClmTuneEntry * cap)()
+    n = ctypes.c_int32(0)
+    torch.cuda.current_stream(base.device).synchronize()
+    rc = _lib.load().clm_autotune(ctx.handle, ctypes.byref(p), ctypes.c_void_p(base.data_ptr()),
+                                  int(sample_pairs), ent, cap, ctypes.byref(n))
+    ctx.check(rc)
+    rows = [dict(variant=e.variant, lag=e.lag, ms=e.ms, pairs=e.pairs,
+                 us_per_kpair=1e6 * e.ms / max(e.pairs, 1)) for e in ent[:min(n.value, cap)]]
+    best = min(rows, key=lambda r: r["ms"] / max(r["pairs"], 1)) if rows else None
+    return rows, best
 
 
 def assemble(K: int, n_theta: int, rec):

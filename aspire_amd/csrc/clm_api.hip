@@ -22,17 +22,35 @@ struct DevBuf {
 
 }  // namespace
 
+struct StackBufs {
+    DevBuf P64, P32, Bop, nrm, astk, recs, ph32, ph64, shiftv, H;
+};
+
+// Pass-1 tiles of one group, kept while the group's shapes repeat.
+struct TileCache {
+    std::vector<long long> key;  // (table index, K, PPW) per member
+    DevBuf tiles;
+    int ntiles = 0;
+};
+
 struct clm_ctx {
     int device = 0;
     int n_cu = 0;
     int lds_max = 65536;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf P64, P32, Bop, nrm, recs, tiles, plist, ctr, ulist, erecs, fblist;
-    DevBuf ph32, ph64, shiftv, H, pf, out_cl1, out_cl2, out_j, out_corr, out_shift, uslots;
+    std::vector<StackBufs> sb;          // per stack of the current call
+    std::vector<TileCache> tcache;      // most recent first
+    DevBuf table, ctr, ulist, erecs, plist, tiles2, fblist;
+    DevBuf pf, out_cl1, out_cl2, out_j, out_corr, out_shift;  // host-buffer form
     hipEvent_t ev[6] = {};
-    std::vector<int4> h_tiles;
-    int tiles_K = -1, tiles_PPW = -1;
+    // autotuned variants: shape key -> (ct, W, lag)
+    std::vector<std::pair<unsigned long long, int4>> tuned;
+    DevBuf tune_tiles;
+    // captured device job of the grouped form, replayed while the signature repeats
+    std::vector<unsigned long long> gsig;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -105,13 +123,13 @@ Variant choose_variant(int opk, int NCT, int Rp, int want_ct, int want_W) {
     for (int i = 0; i < ncts; i++) {
         const int ct = cts[i];
         if (want_ct && ct != want_ct) continue;
-        if (opk == OPK_F32 && ct == 2 && Rp > 64) continue;
+        if (opk == OPK_F32 && ct == 2 && Rp > 32) continue;  // register budget
         if (opk != OPK_F32 && ct == 3 && Rp > 64) continue;  // register budget
         const int NCTp = (NCT + ct - 1) / ct * ct;
         const int CPP = NCTp / ct;
         if (CPP > 8) continue;
         int W = want_W ? want_W : (8 / CPP) * CPP;
-        if (opk == OPK_F32 && !want_W) W = CPP * std::max(1, 4 / CPP);
+        if (opk == OPK_F32 && !want_W) W = CPP * ((4 + CPP - 1) / CPP);  // at least 4 waves
         if (W < CPP || W % CPP || W > 8) continue;
         const double waste = (double)NCTp / NCT;
         if (waste < best_waste - 1e-9) {
@@ -172,6 +190,32 @@ PairFn pick_pair_kernel(int opk, int ct, int Rp, bool sym, bool emit) {
     const int NK = Rp / 16;
     if (opk == OPK_BF16) return sym ? pick_ct<__bf16, true>(ct, NK, emit) : pick_ct<__bf16, false>(ct, NK, emit);
     return sym ? pick_ct<_Float16, true>(ct, NK, emit) : pick_ct<_Float16, false>(ct, NK, emit);
+}
+
+using AstkFn = void (*)(Geom, const float2*, const float2*, int4*);
+
+template <typename OpT, bool SYM>
+AstkFn pick_astk_nk(int NK) {
+    switch (NK) {
+        case 1: return astack_kernel<OpT, 1, SYM>;
+        case 2: return astack_kernel<OpT, 2, SYM>;
+        case 3: return astack_kernel<OpT, 3, SYM>;
+        case 4: return astack_kernel<OpT, 4, SYM>;
+        case 6: return astack_kernel<OpT, 6, SYM>;
+        case 8: return astack_kernel<OpT, 8, SYM>;
+        case 16: return astack_kernel<OpT, 16, SYM>;
+        case 24: return astack_kernel<OpT, 24, SYM>;
+        case 32: return astack_kernel<OpT, 32, SYM>;
+        case 48: return astack_kernel<OpT, 48, SYM>;
+        case 64: return astack_kernel<OpT, 64, SYM>;
+    }
+    return nullptr;
+}
+
+AstkFn pick_astk(int opk, int Rp, bool sym) {
+    if (opk == OPK_F32) return sym ? pick_astk_nk<float, true>(Rp / 2) : pick_astk_nk<float, false>(Rp / 2);
+    if (opk == OPK_BF16) return sym ? pick_astk_nk<__bf16, true>(Rp / 16) : pick_astk_nk<__bf16, false>(Rp / 16);
+    return sym ? pick_astk_nk<_Float16, true>(Rp / 16) : pick_astk_nk<_Float16, false>(Rp / 16);
 }
 
 inline long long pair_slot(long long k1, long long k2, long long K) {
@@ -265,288 +309,611 @@ int upload(clm_ctx* c, DevBuf& b, const std::vector<T>& v) {
     return CLM_OK;
 }
 
-int fill(clm_ctx* c, void* p, int n, int v) {
-    if (n <= 0) return CLM_OK;
-    hipLaunchKernelGGL(fill_i32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (int*)p, n, v);
-    HIPCHK(c, hipGetLastError());
-    return CLM_OK;
-}
-
 float event_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
     return ms;
 }
 
-// Device job: everything from pf (device) to the per-pair records (device).
-int run_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t* d_cl1, int32_t* d_cl2,
-               int32_t* d_j, double* d_corr, double* d_shift, clm_stats* st) {
+// Largest number of pairs given a second (exact-threshold) pass per group
+// and call; uncertain pairs beyond it go to the FP64 exhaustive kernel.
+constexpr int UCAP = 32768;
+
+// One stack of a call (one cryo_clmatrix problem).
+struct StackJob {
+    clm_params p;
+    clm_stack_io io;
     Setup s;
-    int rc = setup(c, p, s);
+    int opk = 0, exact = 0, use64 = 0;
+    Variant v;
+    int lag = 0;
+    long long P = 0;
+    int group = -1;
+};
+
+// Stacks that share one pair-kernel instantiation (or the FP64 kernel).
+struct Group {
+    int opk = 0, exact = 0, ct = 0, Rp = 0, sym = 0, W = 0, CPP = 0, lag = 0;
+    std::vector<int> members;   // job indices
+    int first = 0;              // first table index
+    long long P = 0;            // pairs of all members
+    long long ubase = 0;        // offset of its uncertain list
+    PairFn fn = nullptr, fn2 = nullptr;
+    size_t lds = 0;
+    int occ = 1;
+    const TileCache* tc = nullptr;
+};
+
+struct Plan {
+    std::vector<StackJob> jobs;
+    std::vector<Group> groups;
+    std::vector<int> order;      // table index -> job index
+    std::vector<StackDev> table;
+    long long P = 0;
+    int n_unc_cap = 0;           // entries of the shared pass-2 buffers
+};
+
+int enqueue_fill(clm_ctx* c, void* p, long long n, int v) {
+    if (n <= 0) return CLM_OK;
+    hipLaunchKernelGGL(fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (int*)p, (int)n, v);
+    HIPCHK(c, hipGetLastError());
+    return CLM_OK;
+}
+
+// Validate one stack and derive its geometry and kernel variant.
+// Shape key of the autotuner's table: what the kernel variants' speed depends on.
+unsigned long long tune_key(int opk, const Geom& g) {
+    return (unsigned long long)opk | ((unsigned long long)g.L << 2) | ((unsigned long long)g.Rp << 16) |
+           ((unsigned long long)g.NE << 24) | ((unsigned long long)g.sym << 40);
+}
+
+int plan_stack(clm_ctx* c, const clm_params* p, StackJob& j) {
+    int rc = setup(c, p, j.s);
     if (rc) return rc;
-    Geom& g = s.g;
+    Geom& g = j.s.g;
     const long long K = g.K;
-    const long long P = K * (K - 1) / 2;
-    if (st) {
-        std::memset(st, 0, sizeof *st);
-        st->pairs = P;
-        st->n_eval_shifts = g.NE;
-    }
-    if (P == 0) return CLM_OK;
-    if (P > 0x7fffffffLL) return fail(c, CLM_E_UNSUP, "K too large (%lld pairs)", P);
+    j.P = K * (K - 1) / 2;
+    if (j.P > 0x7fffffffLL) return fail(c, CLM_E_UNSUP, "K too large (%lld pairs)", j.P);
     const int prec = p->precision;
     if (prec < CLM_FP32 || prec > CLM_FP64_REF) return fail(c, CLM_E_ARG, "unknown precision %d", prec);
     if (p->pf_dtype != CLM_C128 && p->pf_dtype != CLM_C64) return fail(c, CLM_E_ARG, "unknown pf dtype %d", p->pf_dtype);
     if (g.Rp < 0) return fail(c, CLM_E_UNSUP, "n_r=%d exceeds the supported 129", g.n_r);
-    const int opk = prec == CLM_BF16_RESCORE ? OPK_BF16 : prec == CLM_FP16_RESCORE ? OPK_F16 : OPK_F32;
-    const bool use64 = prec != CLM_FP32;
+    j.opk = prec == CLM_BF16_RESCORE ? OPK_BF16 : prec == CLM_FP16_RESCORE ? OPK_F16 : OPK_F32;
+    j.exact = prec == CLM_FP64_REF;
+    j.use64 = prec != CLM_FP32;
     const int NCT = (g.L + 31) / 32;
-    Variant v = choose_variant(opk, NCT, g.Rp, p->variant / 100, p->variant % 100);
-    if (!v.ct) return fail(c, CLM_E_UNSUP, "no kernel variant for L=%d (variant %d)", g.L, p->variant);
-    g.NCT = v.NCTp;
-    g.Lp = 32 * v.NCTp;
+    int want_ct = p->variant / 100, want_W = p->variant % 100, want_lag = 1;
+    if (p->variant == 0) {
+        const unsigned long long key = tune_key(j.opk, g);
+        for (const auto& e : c->tuned)
+            if (e.first == key) { want_ct = e.second.x; want_W = e.second.y; want_lag = e.second.z; }
+    }
+    j.v = choose_variant(j.opk, NCT, g.Rp, want_ct, want_W);
+    if (!j.v.ct) return fail(c, CLM_E_UNSUP, "no kernel variant for L=%d (variant %d)", g.L, p->variant);
+    g.NCT = j.v.NCTp;
+    g.Lp = 32 * j.v.NCTp;
     g.NROWS = g.NE * g.L;
     g.NAT = (g.NROWS + 31) / 32;
     if ((long long)g.NROWS * g.Lp > 0xffffffffLL) return fail(c, CLM_E_UNSUP, "map too large");
-    if (st) st->variant = v.ct * 100 + v.W;
-    const size_t osz = opk == OPK_F32 ? 4 : 2;
+    j.lag = (j.v.W % 2 == 0 && want_lag && !(p->reserved & 1)) ? 1 : 0;
+    return CLM_OK;
+}
 
-    // ---- buffers
-    if ((rc = ensure(c, c->P32, (size_t)K * g.L * g.Rp * sizeof(float2)))) return rc;
-    if (use64 && (rc = ensure(c, c->P64, (size_t)K * g.L * g.R * sizeof(double2)))) return rc;
-    if ((rc = ensure(c, c->Bop, (size_t)K * 2 * g.Lp * g.Rp * osz))) return rc;
-    if ((rc = ensure(c, c->nrm, (size_t)K * sizeof(float)))) return rc;
-    if ((rc = ensure(c, c->ctr, 64 * sizeof(unsigned long long)))) return rc;
-    if ((rc = upload(c, c->ph32, s.ph32))) return rc;
-    if ((rc = upload(c, c->ph64, s.ph64))) return rc;
-    if ((rc = upload(c, c->shiftv, s.shifts))) return rc;
-    if ((rc = upload(c, c->H, s.H))) return rc;
+// Allocate and upload everything the device job reads; build the table.
+// Nothing here is captured: the job itself (enqueue_job) only launches kernels.
+int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_io* ios, bool build_tiles = true) {
+    int rc;
+    pl.jobs.resize(n);
+    for (int i = 0; i < n; i++) {
+        StackJob& j = pl.jobs[i];
+        j.p = ps[i];
+        j.io = ios[i];
+        if ((rc = plan_stack(c, &ps[i], j))) {
+            if (n > 1) c->err = "stack " + std::to_string(i) + ": " + c->err;
+            return rc;
+        }
+        if (j.P > 0 && !j.io.pf) return fail(c, CLM_E_ARG, "null pf (stack %d)", i);
+    }
+    // groups: same kernel instantiation (pairs-free stacks need none)
+    for (int i = 0; i < n; i++) {
+        StackJob& j = pl.jobs[i];
+        if (j.P == 0) continue;
+        const Geom& g = j.s.g;
+        const int lag = j.lag;
+        int gi = -1;
+        for (size_t q = 0; q < pl.groups.size(); q++) {
+            const Group& G = pl.groups[q];
+            if (G.exact == j.exact && (j.exact || (G.opk == j.opk && G.ct == j.v.ct && G.Rp == g.Rp &&
+                                                   G.sym == g.sym && G.W == j.v.W && G.CPP == j.v.CPP &&
+                                                   G.lag == lag))) {
+                gi = (int)q;
+                break;
+            }
+        }
+        if (gi < 0) {
+            Group G;
+            G.opk = j.opk; G.exact = j.exact; G.ct = j.v.ct; G.Rp = g.Rp; G.sym = g.sym;
+            G.W = j.v.W; G.CPP = j.v.CPP; G.lag = lag;
+            pl.groups.push_back(G);
+            gi = (int)pl.groups.size() - 1;
+        }
+        pl.groups[gi].members.push_back(i);
+        j.group = gi;
+    }
+    // table in group order; pair_base relative to the group
+    pl.order.clear();
+    pl.P = 0;
+    long long ubase = 0;
+    for (Group& G : pl.groups) {
+        G.first = (int)pl.order.size();
+        G.P = 0;
+        for (int i : G.members) {
+            pl.order.push_back(i);
+            G.P += pl.jobs[i].P;
+        }
+        G.ubase = ubase;
+        if (!G.exact) ubase += G.P;
+        pl.P += G.P;
+    }
+    const int nt = (int)pl.order.size();
+    if (c->sb.size() < (size_t)nt) c->sb.resize(nt);
+    pl.table.assign(nt, StackDev{});
+    for (int ti = 0; ti < nt; ti++) {
+        StackJob& j = pl.jobs[pl.order[ti]];
+        StackBufs& B = c->sb[ti];
+        const Geom& g = j.s.g;
+        const long long K = g.K;
+        const size_t osz = j.opk == OPK_F32 ? 4 : 2;
+        if ((rc = ensure(c, B.P64, (size_t)K * g.L * g.R * sizeof(double2)))) return rc;
+        if ((rc = ensure(c, B.P32, (size_t)K * g.L * g.Rp * sizeof(float2)))) return rc;
+        if ((rc = ensure(c, B.Bop, (size_t)K * 2 * g.Lp * g.Rp * osz))) return rc;
+        if ((rc = ensure(c, B.nrm, (size_t)K * sizeof(float)))) return rc;
+        if (!j.exact) {
+            const size_t tile_bytes = 32 * 2 * (size_t)g.Rp * osz;
+            if ((rc = ensure(c, B.astk, (size_t)K * g.NAT * tile_bytes))) return rc;
+            if ((rc = ensure(c, B.recs, (size_t)j.P * sizeof(PairRec)))) return rc;
+        }
+        if ((rc = upload(c, B.ph32, j.s.ph32))) return rc;
+        if ((rc = upload(c, B.ph64, j.s.ph64))) return rc;
+        if ((rc = upload(c, B.shiftv, j.s.shifts))) return rc;
+        if ((rc = upload(c, B.H, j.s.H))) return rc;
+        StackDev& d = pl.table[ti];
+        d.g = g;
+        d.astk = B.astk.p;
+        d.Bop = B.Bop.p;
+        d.nrm = (const float*)B.nrm.p;
+        d.recs = (PairRec*)B.recs.p;
+        d.P32 = (const float2*)B.P32.p;
+        d.P64 = (const double2*)B.P64.p;
+        d.ph32 = (const float2*)B.ph32.p;
+        d.ph64 = (const double2*)B.ph64.p;
+        d.shiftv = (const double*)B.shiftv.p;
+        d.cl1 = j.io.cl1; d.cl2 = j.io.cl2; d.jidx = j.io.jidx; d.corr = j.io.corr; d.shift1d = j.io.shift1d;
+        const float u = j.opk == OPK_BF16 ? 0x1p-8f : j.opk == OPK_F16 ? 0x1p-11f : 0x1p-24f;
+        d.coef = (2.0f * u + u * u + (2.0f * g.Rp + 16.0f) * 0x1p-24f) * 1.01f;
+        d.sub = j.opk == OPK_F16 ? 0x1p-24f * std::sqrt(2.0f * g.Rp) : 0.0f;
+        d.tau_t = j.use64 ? (float)(0.5 * j.p.tie_margin) : 0.0f;
+        d.tau = (float)j.p.tie_margin;
+        d.fp64 = j.use64;
+        d.CPP = j.v.CPP;
+        d.npairs = j.P;
+    }
+    for (Group& G : pl.groups) {
+        long long base = 0;
+        for (int ti = G.first; ti < G.first + (int)G.members.size(); ti++) {
+            pl.table[ti].pair_base = base;
+            base += pl.table[ti].npairs;
+        }
+    }
+    if ((rc = upload(c, c->table, pl.table))) return rc;
+    // shared buffers: counters (16 globals + 4 per group), lists, pass-2 tiles
+    if ((rc = ensure(c, c->ctr, (16 + 4 * pl.groups.size()) * sizeof(unsigned long long)))) return rc;
+    if ((rc = ensure(c, c->ulist, (size_t)std::max<long long>(ubase, 1) * sizeof(SlotRef)))) return rc;
+    if ((rc = ensure(c, c->fblist, (size_t)std::max<long long>(pl.P, 1) * sizeof(SlotRef)))) return rc;
+    long long gmax = 0;
+    for (const Group& G : pl.groups)
+        if (!G.exact) gmax = std::max(gmax, G.P);
+    pl.n_unc_cap = (int)std::min<long long>(gmax, UCAP);
+    if ((rc = ensure(c, c->erecs, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(EmitRec)))) return rc;
+    if ((rc = ensure(c, c->plist, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(int4)))) return rc;
+    if ((rc = ensure(c, c->tiles2, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(int4)))) return rc;
+    // kernels and pass-1 tiles per group
+    for (Group& G : pl.groups) {
+        if (G.exact) continue;
+        G.fn = pick_pair_kernel(G.opk, G.ct, G.Rp, G.sym != 0, false);
+        G.fn2 = pick_pair_kernel(G.opk, G.ct, G.Rp, G.sym != 0, true);
+        if (!G.fn || !G.fn2) return fail(c, CLM_E_UNSUP, "no kernel for ct=%d Rp=%d", G.ct, G.Rp);
+        if (G.W < 4 || G.W > 8) return fail(c, CLM_E_UNSUP, "workgroup of %d waves", G.W);
+        G.lds = pair_lds_bytes(G.W);
+        G.occ = 0;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&G.occ, (const void*)G.fn, 64 * G.W, G.lds));
+        if (G.occ < 1) G.occ = 1;
+        if (!build_tiles) continue;
+        const int PPW = G.W / G.CPP;
+        std::vector<long long> key;
+        for (int ti = G.first; ti < G.first + (int)G.members.size(); ti++) {
+            key.push_back(ti);
+            key.push_back(pl.table[ti].g.K);
+            key.push_back(PPW);
+        }
+        bool hit = false;
+        for (const TileCache& tc : c->tcache) hit = hit || tc.key == key;
+        if (!hit) {
+            std::vector<int4> h;
+            for (int ti = G.first; ti < G.first + (int)G.members.size(); ti++) {
+                const long long K = pl.table[ti].g.K;
+                for (long long k1 = 0; k1 + 1 < K; k1++)
+                    for (long long k2 = k1 + 1; k2 < K; k2 += PPW)
+                        h.push_back(make_int4((int)k1, (int)k2, (int)std::min<long long>(PPW, K - k2) | (ti << 4),
+                                              (int)pair_slot(k1, k2, K)));
+            }
+            if (c->tcache.size() >= std::max<size_t>(4, pl.groups.size())) {
+                release(c->tcache.back().tiles);
+                c->tcache.pop_back();
+            }
+            c->tcache.insert(c->tcache.begin(), TileCache{});
+            TileCache& tc = c->tcache.front();
+            tc.key = key;
+            tc.ntiles = (int)h.size();
+            if ((rc = upload(c, tc.tiles, h))) return rc;
+            HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released on return
+        }
+    }
+    // tile caches can move while inserting: resolve pointers afterwards
+    for (Group& G : pl.groups) {
+        if (G.exact || !build_tiles) continue;
+        const int PPW = G.W / G.CPP;
+        std::vector<long long> key;
+        for (int ti = G.first; ti < G.first + (int)G.members.size(); ti++) {
+            key.push_back(ti);
+            key.push_back(pl.table[ti].g.K);
+            key.push_back(PPW);
+        }
+        for (const TileCache& tc : c->tcache)
+            if (tc.key == key) G.tc = &tc;
+        if (!G.tc) return fail(c, CLM_E_UNSUP, "tile cache too small for %zu groups", pl.groups.size());
+    }
+    return CLM_OK;
+}
+
+// Per-stack preprocessing (lines, filter, normalization, operand planes) and
+// A-stacks.
+int enqueue_prep(clm_ctx* c, const Plan& pl) {
+    int rc;
     unsigned long long* ctr = (unsigned long long*)c->ctr.p;
-    // ctr layout (u64): 0 n_small, 1 fp64 candidates, 2 tile counter (as int),
-    // 3 uncertain count (int), 4 fallback count (int)
-    if ((rc = fill(c, ctr, 2 * 64, 0))) return rc;
-    if ((rc = fill(c, c->nrm.p, (int)K, 0))) return rc;
-
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    {
-        const long long lines = K * g.Lp;
-        const int bs = 256;
-        const unsigned nb = (unsigned)((lines + bs - 1) / bs);
-        if (p->pf_dtype == CLM_C128)
-            hipLaunchKernelGGL(prep_kernel<double>, dim3(nb), dim3(bs), 0, c->stream, (const double*)d_pf, g,
-                               (const double*)c->H.p, use64 ? (double2*)c->P64.p : nullptr, (float2*)c->P32.p,
-                               c->Bop.p, opk, (unsigned*)c->nrm.p, ctr + 0);
+    for (size_t ti = 0; ti < pl.order.size(); ti++) {
+        const StackJob& j = pl.jobs[pl.order[ti]];
+        const StackBufs& B = c->sb[ti];
+        const Geom& g = j.s.g;
+        if ((rc = enqueue_fill(c, B.nrm.p, g.K, 0))) return rc;
+        const long long lines = (long long)g.K * g.Lp;
+        const unsigned nb = (unsigned)((lines + 255) / 256);
+        if (j.p.pf_dtype == CLM_C128)
+            hipLaunchKernelGGL(prep_kernel<double>, dim3(nb), dim3(256), 0, c->stream, (const double*)j.io.pf, g,
+                               (const double*)B.H.p, (double2*)B.P64.p, (float2*)B.P32.p, B.Bop.p, j.opk,
+                               (unsigned*)B.nrm.p, ctr + 0);
         else
-            hipLaunchKernelGGL(prep_kernel<float>, dim3(nb), dim3(bs), 0, c->stream, (const float*)d_pf, g,
-                               (const double*)c->H.p, use64 ? (double2*)c->P64.p : nullptr, (float2*)c->P32.p,
-                               c->Bop.p, opk, (unsigned*)c->nrm.p, ctr + 0);
+            hipLaunchKernelGGL(prep_kernel<float>, dim3(nb), dim3(256), 0, c->stream, (const float*)j.io.pf, g,
+                               (const double*)B.H.p, (double2*)B.P64.p, (float2*)B.P32.p, B.Bop.p, j.opk,
+                               (unsigned*)B.nrm.p, ctr + 0);
+        HIPCHK(c, hipGetLastError());
+        if (!j.exact) {
+            const size_t osz = j.opk == OPK_F32 ? 4 : 2;
+            const long long nch = (long long)g.K * g.NAT * (long long)(32 * 2 * (size_t)g.Rp * osz / 16);
+            AstkFn af = pick_astk(j.opk, g.Rp, g.sym != 0);
+            if (!af) return fail(c, CLM_E_UNSUP, "no A-stack kernel for Rp=%d", g.Rp);
+            hipLaunchKernelGGL(af, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, c->stream, g,
+                               (const float2*)B.P32.p, (const float2*)B.ph32.p, (int4*)B.astk.p);
+            HIPCHK(c, hipGetLastError());
+        }
+    }
+    return CLM_OK;
+}
+
+// The device job: kernel launches only (no copies, no host waits), so that it
+// can be captured into a HIP graph.  With `timed`, events split it into
+// preprocessing, pair contraction and re-scoring.
+int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
+    int rc;
+    unsigned long long* ctr = (unsigned long long*)c->ctr.p;
+    // ctr (u64): 0 small-norm lines, 1 FP64 candidates, 4 fallback count (int),
+    // 6 cells re-scored, 7 second-pass candidates; per group at 16+4g:
+    // 0 pass-1 tile counter, 1 uncertain count, 2 pass-2 tile counter, 3 pass-2 tiles
+    if ((rc = enqueue_fill(c, ctr, 2 * (16 + 4 * (long long)pl.groups.size()), 0))) return rc;
+    const StackDev* dtab = (const StackDev*)c->table.p;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = enqueue_prep(c, pl))) return rc;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    // pass 1, one persistent launch per group
+    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+        const Group& G = pl.groups[gi];
+        if (G.exact) {
+            Exact64Args ea{};
+            ea.stacks = dtab;
+            ea.stack0 = G.first;
+            ea.nstacks = (int)G.members.size();
+            ea.list = nullptr;
+            ea.n_dev = nullptr;
+            ea.n_all = G.P;
+            const unsigned nb = (unsigned)std::min<long long>(G.P, (long long)c->n_cu * 8);
+            hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
+            HIPCHK(c, hipGetLastError());
+            continue;
+        }
+        unsigned long long* cg = ctr + 16 + 4 * gi;
+        PairArgs pa{};
+        pa.stacks = dtab;
+        pa.tiles = (const int4*)G.tc->tiles.p;
+        pa.ntiles = G.tc->ntiles;
+        pa.ntiles_dev = nullptr;
+        pa.counter = (int*)(cg + 0);
+        pa.plist = nullptr;
+        pa.erecs = nullptr;
+        pa.CPP = G.CPP;
+        pa.lag = G.lag;
+        const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, G.tc->ntiles);
+        hipLaunchKernelGGL(G.fn, dim3(grid), dim3(64 * G.W), G.lds, c->stream, pa);
         HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-
-    int n_unc = 0, n_fb = 0;
-    if (prec == CLM_FP64_REF) {
-        Exact64Args ea{};
-        ea.g = g;
-        ea.P64 = (const double2*)c->P64.p;
-        ea.ph64 = (const double2*)c->ph64.p;
-        ea.shiftv = (const double*)c->shiftv.p;
-        ea.list = nullptr;
-        ea.n = (int)P;
-        ea.K = (int)K;
-        ea.cl1 = d_cl1; ea.cl2 = d_cl2; ea.jidx = d_j; ea.corr = d_corr; ea.shift1d = d_shift;
-        const unsigned nb = (unsigned)std::min<long long>(P, (long long)c->n_cu * 8);
-        hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    } else {
-        // ---- pass 1 tiles: rows of the pair triangle cut into PPW pairs
-        const int PPW = v.W / v.CPP;
-        if (c->tiles_K != (int)K || c->tiles_PPW != PPW) {
-            c->h_tiles.clear();
-            for (long long k1 = 0; k1 + 1 < K; k1++)
-                for (long long k2 = k1 + 1; k2 < K; k2 += PPW)
-                    c->h_tiles.push_back(make_int4((int)k1, (int)k2, (int)std::min<long long>(PPW, K - k2),
-                                                   (int)pair_slot(k1, k2, K)));
-            if ((rc = upload(c, c->tiles, c->h_tiles))) return rc;
-            c->tiles_K = (int)K;
-            c->tiles_PPW = PPW;
-        }
-        const int ntiles = (int)c->h_tiles.size();
-        if ((rc = ensure(c, c->recs, (size_t)P * sizeof(PairRec)))) return rc;
-        if ((rc = ensure(c, c->ulist, (size_t)P * sizeof(int)))) return rc;
-
-        const float u = opk == OPK_BF16 ? 0x1p-8f : opk == OPK_F16 ? 0x1p-11f : 0x1p-24f;
-        const float sub = opk == OPK_F16 ? 0x1p-24f * std::sqrt(2.0f * g.Rp) : 0.0f;
-        PairArgs pa{};
-        pa.g = g;
-        pa.P32 = (const float2*)c->P32.p;
-        pa.Bop = c->Bop.p;
-        pa.ph32 = (const float2*)c->ph32.p;
-        pa.nrm = (const float*)c->nrm.p;
-        pa.tiles = (const int4*)c->tiles.p;
-        pa.ntiles = ntiles;
-        pa.counter = (int*)(ctr + 2);
-        pa.plist = nullptr;
-        pa.recs = (PairRec*)c->recs.p;
-        pa.erecs = nullptr;
-        pa.CPP = v.CPP;
-        pa.coef = (2.0f * u + u * u + (2.0f * g.Rp + 16.0f) * 0x1p-24f) * 1.01f;
-        pa.sub = sub;
-        pa.tau_t = use64 ? (float)(0.5 * p->tie_margin) : 0.0f;
-        PairFn fn = pick_pair_kernel(opk, v.ct, g.Rp, g.sym != 0, false);
-        PairFn fn2 = pick_pair_kernel(opk, v.ct, g.Rp, g.sym != 0, true);
-        if (!fn || !fn2) return fail(c, CLM_E_UNSUP, "no kernel for ct=%d Rp=%d", v.ct, g.Rp);
-        const int osz_lds = opk == OPK_F32 ? 4 : 2;
-        pa.stage = pair_lds_bytes(g.Rp, osz_lds, v.W, g.NE, g.L, 1) <= (size_t)c->lds_max ? 1 : 0;
-        const size_t lds = pair_lds_bytes(g.Rp, osz_lds, v.W, g.NE, g.L, pa.stage);
-        if (lds > 65536) {
-            HIPCHK(c, hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            HIPCHK(c, hipFuncSetAttribute((const void*)fn2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        }
-        int occ = 0;
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)fn, 64 * v.W, lds));
-        if (occ < 1) occ = 1;
-        const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * occ, ntiles);
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * v.W), lds, c->stream, pa);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-
-        // ---- re-score
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    // re-score; uncertain pairs get the exact-threshold second pass
+    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+        const Group& G = pl.groups[gi];
+        if (G.exact) continue;
+        unsigned long long* cg = ctr + 16 + 4 * gi;
+        SlotRef* ul = (SlotRef*)c->ulist.p + G.ubase;
         RescoreArgs ra{};
-        ra.g = g;
-        ra.P32 = (const float2*)c->P32.p;
-        ra.P64 = use64 ? (const double2*)c->P64.p : nullptr;
-        ra.ph32 = (const float2*)c->ph32.p;
-        ra.ph64 = (const double2*)c->ph64.p;
-        ra.shiftv = (const double*)c->shiftv.p;
-        ra.recs = (const PairRec*)c->recs.p;
+        ra.stacks = dtab;
+        ra.stack0 = G.first;
+        ra.nstacks = (int)G.members.size();
         ra.mode = 0;
-        ra.n_items = (int)P;
-        ra.fp64 = use64 ? 1 : 0;
-        ra.tau = (float)p->tie_margin;
-        ra.unc_count = (int*)(ctr + 3);
-        ra.unc_list = (int*)c->ulist.p;
+        ra.n_items = G.P;
+        ra.unc_count = (int*)(cg + 1);
+        ra.unc_list = ul;
         ra.fb_count = (int*)(ctr + 4);
+        ra.fb_list = (SlotRef*)c->fblist.p;
         ra.n64 = ctr + 1;
         ra.ncell = ctr + 6;
         ra.nemit = ctr + 7;
-        ra.cl1 = d_cl1; ra.cl2 = d_cl2; ra.jidx = d_j; ra.corr = d_corr; ra.shift1d = d_shift;
-        const unsigned rgrid = (unsigned)std::min<long long>((P + 3) / 4, (long long)c->n_cu * 16);
+        const unsigned rgrid = (unsigned)std::min<long long>((G.P + 3) / 4, (long long)c->n_cu * 16);
         hipLaunchKernelGGL(rescore_kernel, dim3(rgrid), dim3(256), 0, c->stream, ra);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpyAsync(&n_unc, ctr + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-
-        // ---- second pass on uncertain pairs, in batches
-        std::vector<int> fb_all;
-        if (n_unc > 0) {
-            std::vector<int> uslots(n_unc);
-            HIPCHK(c, hipMemcpy(uslots.data(), c->ulist.p, sizeof(int) * n_unc, hipMemcpyDeviceToHost));
-            std::sort(uslots.begin(), uslots.end());
-            const int BATCH = 131072;
-            for (int b0 = 0; b0 < n_unc; b0 += BATCH) {
-                const int nb = std::min(BATCH, n_unc - b0);
-                std::vector<int4> tl, pl;
-                std::vector<int> order(nb);
-                int cur_k1 = -1, cnt = 0;
-                for (int i = 0; i < nb; i++) {
-                    int k1, k2;
-                    slot_pair_host(uslots[b0 + i], K, k1, k2);
-                    if (k1 != cur_k1 || cnt == PPW) {
-                        tl.push_back(make_int4(k1, (int)pl.size(), 0, -1));
-                        cur_k1 = k1;
-                        cnt = 0;
-                    }
-                    pl.push_back(make_int4(k2, uslots[b0 + i], i, 0));
-                    order[i] = uslots[b0 + i];
-                    tl.back().z++;
-                    cnt++;
-                }
-                if ((rc = upload(c, c->plist, pl))) return rc;
-                if ((rc = upload(c, c->uslots, order))) return rc;
-                if ((rc = ensure(c, c->erecs, (size_t)nb * sizeof(EmitRec)))) return rc;
-                DevBuf tb;
-                if ((rc = upload(c, tb, tl))) return rc;
-                if ((rc = fill(c, ctr + 2, 2, 0))) { release(tb); return rc; }
-                pa.tiles = (const int4*)tb.p;
-                pa.ntiles = (int)tl.size();
-                pa.plist = (const int4*)c->plist.p;
-                pa.erecs = (EmitRec*)c->erecs.p;
-                const unsigned g2 = (unsigned)std::min<long long>((long long)c->n_cu * occ, (long long)tl.size());
-                hipLaunchKernelGGL(fn2, dim3(g2), dim3(64 * v.W), lds, c->stream, pa);
-                hipError_t le = hipGetLastError();
-                if (le != hipSuccess) { release(tb); return fail(c, CLM_E_HIP, "pair pass 2: %s", hipGetErrorString(le)); }
-                ra.mode = 1;
-                ra.n_items = nb;
-                ra.erecs = (const EmitRec*)c->erecs.p;
-                ra.ulist = (const int*)c->uslots.p;
-                if ((rc = ensure(c, c->fblist, (size_t)n_unc * sizeof(int)))) { release(tb); return rc; }
-                ra.fb_list = (int*)c->fblist.p;
-                if ((rc = fill(c, ctr + 4, 2, 0))) { release(tb); return rc; }
-                const unsigned rg2 = (unsigned)std::min<long long>((nb + 3) / 4, (long long)c->n_cu * 16);
-                hipLaunchKernelGGL(rescore_kernel, dim3(rg2), dim3(256), 0, c->stream, ra);
-                le = hipGetLastError();
-                if (le != hipSuccess) { release(tb); return fail(c, CLM_E_HIP, "rescore pass 2: %s", hipGetErrorString(le)); }
-                int nfb = 0;
-                HIPCHK(c, hipMemcpyAsync(&nfb, ctr + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                release(tb);
-                if (nfb > 0) {
-                    std::vector<int> fb(nfb);
-                    HIPCHK(c, hipMemcpy(fb.data(), c->fblist.p, sizeof(int) * nfb, hipMemcpyDeviceToHost));
-                    fb_all.insert(fb_all.end(), fb.begin(), fb.end());
-                }
-            }
-        }
-        n_fb = (int)fb_all.size();
-        if (n_fb > 0) {
-            if (!use64 && (rc = ensure(c, c->P64, (size_t)K * g.L * g.R * sizeof(double2)))) return rc;
-            if (!use64) {
-                // plain FP32 runs do not keep FP64 rays; rebuild them for the fallback
-                const long long lines = K * g.Lp;
-                const unsigned nb = (unsigned)((lines + 255) / 256);
-                if (p->pf_dtype == CLM_C128)
-                    hipLaunchKernelGGL(prep_kernel<double>, dim3(nb), dim3(256), 0, c->stream, (const double*)d_pf, g,
-                                       (const double*)c->H.p, (double2*)c->P64.p, (float2*)c->P32.p, c->Bop.p, opk,
-                                       (unsigned*)c->nrm.p, ctr + 5);
-                else
-                    hipLaunchKernelGGL(prep_kernel<float>, dim3(nb), dim3(256), 0, c->stream, (const float*)d_pf, g,
-                                       (const double*)c->H.p, (double2*)c->P64.p, (float2*)c->P32.p, c->Bop.p, opk,
-                                       (unsigned*)c->nrm.p, ctr + 5);
-                HIPCHK(c, hipGetLastError());
-            }
-            if ((rc = upload(c, c->fblist, fb_all))) return rc;
-            Exact64Args ea{};
-            ea.g = g;
-            ea.P64 = (const double2*)c->P64.p;
-            ea.ph64 = (const double2*)c->ph64.p;
-            ea.shiftv = (const double*)c->shiftv.p;
-            ea.list = (const int*)c->fblist.p;
-            ea.n = n_fb;
-            ea.K = (int)K;
-            ea.cl1 = d_cl1; ea.cl2 = d_cl2; ea.jidx = d_j; ea.corr = d_corr; ea.shift1d = d_shift;
-            const unsigned nb = (unsigned)std::min<long long>(n_fb, (long long)c->n_cu * 8);
-            hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
-            HIPCHK(c, hipGetLastError());
-        }
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        const int cap = (int)std::min<long long>(G.P, pl.n_unc_cap);
+        hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((G.P + 255) / 256, 1024)),
+                           dim3(256), 0, c->stream, dtab, (const int*)(cg + 1), (const SlotRef*)ul, cap,
+                           (int4*)c->tiles2.p, (int4*)c->plist.p, (int*)(cg + 3), (int*)(ctr + 4),
+                           (SlotRef*)c->fblist.p);
+        HIPCHK(c, hipGetLastError());
+        PairArgs pa{};
+        pa.stacks = dtab;
+        pa.tiles = (const int4*)c->tiles2.p;
+        pa.ntiles = 0;
+        pa.ntiles_dev = (const int*)(cg + 3);
+        pa.counter = (int*)(cg + 2);
+        pa.plist = (const int4*)c->plist.p;
+        pa.erecs = (EmitRec*)c->erecs.p;
+        pa.CPP = G.CPP;
+        pa.lag = G.lag;
+        const unsigned g2 = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, cap);
+        hipLaunchKernelGGL(G.fn2, dim3(g2), dim3(64 * G.W), G.lds, c->stream, pa);
+        HIPCHK(c, hipGetLastError());
+        ra.mode = 1;
+        ra.n_items = 0;
+        ra.n_items_dev = (const int*)(cg + 1);
+        ra.item_cap = cap;
+        ra.ulist = ul;
+        ra.erecs = (const EmitRec*)c->erecs.p;
+        const unsigned rg2 = (unsigned)std::min<long long>((cap + 3) / 4, (long long)c->n_cu * 16);
+        hipLaunchKernelGGL(rescore_kernel, dim3(rg2), dim3(256), 0, c->stream, ra);
+        HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (st) {
-        unsigned long long h[8] = {0};
-        HIPCHK(c, hipMemcpy(h, ctr, sizeof h, hipMemcpyDeviceToHost));
-        st->small_norm_lines = (int64_t)h[0];
-        st->fp64_candidates = (int64_t)h[1];
-        st->cells_rescored = (int64_t)h[6];
-        st->emitted = (int64_t)h[7];
-        st->uncertain_pairs = n_unc;
-        st->fallback_pairs = prec == CLM_FP64_REF ? P : n_fb;
+    // pairs no pass could settle: FP64 exhaustive search
+    if (pl.P > 0) {
+        Exact64Args ea{};
+        ea.stacks = dtab;
+        ea.list = (const SlotRef*)c->fblist.p;
+        ea.n_dev = (const int*)(ctr + 4);
+        const unsigned nb = (unsigned)std::min<long long>(pl.P, (long long)c->n_cu * 8);
+        hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    return CLM_OK;
+}
+
+int read_stats(clm_ctx* c, const Plan& pl, bool timed, clm_stats* st) {
+    if (!st) return CLM_OK;
+    std::memset(st, 0, sizeof *st);
+    for (const StackJob& j : pl.jobs) st->pairs += j.P;
+    if (!pl.jobs.empty()) {
+        st->n_eval_shifts = pl.jobs[0].s.g.NE;
+        st->variant = pl.jobs[0].v.ct * 100 + pl.jobs[0].v.W;
+    }
+    if (pl.P == 0) return CLM_OK;
+    std::vector<unsigned long long> h(16 + 4 * pl.groups.size());
+    HIPCHK(c, hipMemcpy(h.data(), c->ctr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    auto as_int = [](unsigned long long x) { return (int64_t)(int)(unsigned)(x & 0xffffffffu); };
+    st->small_norm_lines = (int64_t)h[0];
+    st->fp64_candidates = (int64_t)h[1];
+    st->cells_rescored = (int64_t)h[6];
+    st->emitted = (int64_t)h[7];
+    st->fallback_pairs = as_int(h[4]);
+    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+        if (pl.groups[gi].exact) st->fallback_pairs += pl.groups[gi].P;
+        else st->uncertain_pairs += as_int(h[16 + 4 * gi + 1]);
+    }
+    if (timed) {
         st->ms_prep = event_ms(c->ev[0], c->ev[1]);
         st->ms_pairs = event_ms(c->ev[1], c->ev[2]);
         st->ms_rescore = event_ms(c->ev[2], c->ev[3]);
         st->ms_total = event_ms(c->ev[0], c->ev[3]);
+    }
+    return CLM_OK;
+}
+
+void drop_graph(clm_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->gsig.clear();
+}
+
+// Every argument the captured job depends on, buffer addresses included.
+std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
+    std::vector<unsigned long long> s;
+    auto add = [&](const void* p, size_t n) {
+        const size_t w = (n + 7) / 8;
+        const size_t o = s.size();
+        s.resize(o + w, 0ull);
+        std::memcpy(s.data() + o, p, n);
+    };
+    for (const StackJob& j : pl.jobs) {
+        add(&j.p, sizeof j.p);
+        add(&j.io, sizeof j.io);
+    }
+    add(pl.table.data(), pl.table.size() * sizeof(StackDev));
+    for (size_t ti = 0; ti < pl.order.size(); ti++) {
+        const StackBufs& B = c->sb[ti];
+        const void* sp[] = {B.P64.p, B.P32.p, B.Bop.p, B.nrm.p, B.astk.p, B.recs.p, B.ph32.p, B.ph64.p, B.shiftv.p, B.H.p};
+        add(sp, sizeof sp);
+    }
+    const void* bufs[] = {c->table.p, c->ctr.p, c->ulist.p, c->erecs.p, c->plist.p, c->tiles2.p, c->fblist.p};
+    add(bufs, sizeof bufs);
+    for (const Group& G : pl.groups) {
+        const void* t = G.tc ? G.tc->tiles.p : nullptr;
+        add(&t, sizeof t);
+    }
+    return s;
+}
+
+// Device job of n stacks (results in device memory).  With `graph`, the job
+// is captured once and the graph replayed while the signature repeats.
+int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st) {
+    Plan pl;
+    int rc = prepare(c, pl, n, ps, ios);
+    if (rc) return rc;
+    if (pl.P == 0) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return read_stats(c, pl, false, st);
+    }
+    if (!graph) {
+        if ((rc = enqueue_job(c, pl, true))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return read_stats(c, pl, true, st);
+    }
+    std::vector<unsigned long long> sig = signature(c, pl);
+    if (!c->gexec || sig != c->gsig) {
+        drop_graph(c);
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // uploads done before capture
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_job(c, pl, false);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, CLM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        c->graph = gr;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->gsig = sig;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    rc = read_stats(c, pl, false, st);
+    if (st) st->ms_total = event_ms(c->ev[0], c->ev[3]);
+    return rc;
+}
+
+// Candidate variants of a shape: every (CT, W) the kernels support, W a
+// multiple of the waves per pair between 4 and 8.
+std::vector<std::pair<int, int>> candidate_variants(int opk, int NCT, int Rp) {
+    std::vector<std::pair<int, int>> out;
+    for (int ct = 1; ct <= 3; ct++) {
+        if (opk == OPK_F32 && (ct == 3 || (ct == 2 && Rp > 32))) continue;
+        if (opk != OPK_F32 && ct == 3 && Rp > 64) continue;
+        const int CPP = (NCT + ct - 1) / ct;
+        if (CPP > 8) continue;
+        for (int W = CPP; W <= 8; W += CPP)
+            if (W >= 4) out.push_back({ct, W});
+    }
+    return out;
+}
+
+int autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int sample_pairs, clm_tune_entry* out,
+             int max_out, int* n_out) {
+    int rc;
+    if (n_out) *n_out = 0;
+    StackJob base;
+    if ((rc = plan_stack(c, p, base))) return rc;
+    if (base.exact || base.P == 0) return CLM_OK;  // nothing to choose
+    if (!d_pf) return fail(c, CLM_E_ARG, "null pf");
+    const Geom& g0 = base.s.g;
+    const long long Np = std::min<long long>(base.P, sample_pairs > 0 ? sample_pairs : 200000);
+    auto cands = candidate_variants(base.opk, (g0.L + 31) / 32, g0.Rp);
+    double best = 1e30;
+    int4 best_v = make_int4(0, 0, 0, 0);
+    int n = 0;
+    for (const auto& cw : cands) {
+        for (int lag = (cw.second % 2 == 0) ? 1 : 0; lag >= 0; lag--) {
+            clm_params q = *p;
+            q.variant = cw.first * 100 + cw.second;
+            q.reserved = (p->reserved & ~1) | (lag ? 0 : 1);
+            const clm_stack_io io{d_pf, nullptr, nullptr, nullptr, nullptr, nullptr};
+            Plan pl;
+            if ((rc = prepare(c, pl, 1, &q, &io, false))) return rc;
+            const Group& G = pl.groups[0];
+            const long long K = g0.K;
+            const int PPW = G.W / G.CPP;
+            std::vector<int4> h;
+            long long taken = 0;
+            for (long long k1 = 0; k1 + 1 < K && taken < Np; k1++)
+                for (long long k2 = k1 + 1; k2 < K && taken < Np; k2 += PPW) {
+                    const int cnt = (int)std::min<long long>({(long long)PPW, K - k2, Np - taken});
+                    h.push_back(make_int4((int)k1, (int)k2, cnt, (int)pair_slot(k1, k2, K)));
+                    taken += cnt;
+                }
+            if ((rc = upload(c, c->tune_tiles, h))) return rc;
+            HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released on return
+            if ((rc = enqueue_fill(c, c->ctr.p, 2 * 20, 0))) return rc;
+            if ((rc = enqueue_prep(c, pl))) return rc;
+            PairArgs pa{};
+            pa.stacks = (const StackDev*)c->table.p;
+            pa.tiles = (const int4*)c->tune_tiles.p;
+            pa.ntiles = (int)h.size();
+            pa.counter = (int*)((unsigned long long*)c->ctr.p + 16);
+            pa.CPP = G.CPP;
+            pa.lag = G.lag;
+            const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, (long long)h.size());
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; rep++) {  // first launch warms caches and code
+                if ((rc = enqueue_fill(c, pa.counter, 1, 0))) return rc;
+                HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+                hipLaunchKernelGGL(G.fn, dim3(grid), dim3(64 * G.W), G.lds, c->stream, pa);
+                HIPCHK(c, hipGetLastError());
+                HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                ms = event_ms(c->ev[4], c->ev[5]);
+            }
+            const double per = (double)ms / (double)taken;
+            if (out && n < max_out) out[n] = clm_tune_entry{q.variant, G.lag, (double)ms, (int64_t)taken};
+            n++;
+            if (per < best) {
+                best = per;
+                best_v = make_int4(cw.first, cw.second, G.lag, 0);
+            }
+        }
+    }
+    if (n_out) *n_out = n;
+    if (best_v.x) {
+        const unsigned long long key = tune_key(base.opk, base.s.g);
+        bool found = false;
+        for (auto& e : c->tuned)
+            if (e.first == key) { e.second = best_v; found = true; }
+        if (!found) c->tuned.push_back({key, best_v});
     }
     return CLM_OK;
 }
@@ -602,9 +969,15 @@ void clm_destroy(clm_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->P64, &c->P32, &c->Bop, &c->nrm, &c->recs, &c->tiles, &c->plist, &c->ctr,
-                      &c->ulist, &c->erecs, &c->fblist, &c->ph32, &c->ph64, &c->shiftv, &c->H, &c->pf,
-                      &c->out_cl1, &c->out_cl2, &c->out_j, &c->out_corr, &c->out_shift, &c->uslots};
+    drop_graph(c);
+    for (StackBufs& B : c->sb) {
+        DevBuf* sbufs[] = {&B.P64, &B.P32, &B.Bop, &B.nrm, &B.astk, &B.recs, &B.ph32, &B.ph64, &B.shiftv, &B.H};
+        for (DevBuf* b : sbufs) release(*b);
+    }
+    for (TileCache& tc : c->tcache) release(tc.tiles);
+    release(c->tune_tiles);
+    DevBuf* bufs[] = {&c->table, &c->ctr, &c->ulist, &c->erecs, &c->plist, &c->tiles2, &c->fblist, &c->pf,
+                      &c->out_cl1, &c->out_cl2, &c->out_j, &c->out_corr, &c->out_shift};
     for (DevBuf* b : bufs) release(*b);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -620,7 +993,26 @@ int clm_clmatrix_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32
     if (!p) return fail(c, CLM_E_ARG, "null params");
     if (!d_pf && p->K > 1) return fail(c, CLM_E_ARG, "null pf");
     HIPCHK(c, hipSetDevice(c->device));
-    return run_device(c, p, d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d, st);
+    const clm_stack_io io{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
+    return run_job(c, 1, p, &io, false, st);
+}
+
+int clm_autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t sample_pairs, clm_tune_entry* entries,
+                 int32_t max_entries, int32_t* n_entries) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    if (max_entries < 0 || (max_entries > 0 && !entries)) return fail(c, CLM_E_ARG, "invalid entry buffer");
+    HIPCHK(c, hipSetDevice(c->device));
+    return autotune(c, p, d_pf, sample_pairs, entries, max_entries, n_entries);
+}
+
+int clm_clmatrix_grouped(clm_ctx* c, int32_t n, const clm_params* p, const clm_stack_io* io, int32_t flags,
+                         clm_stats* st) {
+    if (!c) return CLM_E_ARG;
+    if (n < 0 || (n > 0 && (!p || !io))) return fail(c, CLM_E_ARG, "invalid stack list");
+    if (flags & ~CLM_GROUP_GRAPH) return fail(c, CLM_E_ARG, "unknown flags 0x%x", flags);
+    HIPCHK(c, hipSetDevice(c->device));
+    return run_job(c, n, p, io, (flags & CLM_GROUP_GRAPH) != 0, st);
 }
 
 int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, int32_t* cl2, int32_t* jidx,
@@ -641,8 +1033,9 @@ int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, 
     if ((rc = ensure(c, c->out_corr, P * 8))) return rc;
     if ((rc = ensure(c, c->out_shift, P * 8))) return rc;
     if (pf_bytes) HIPCHK(c, hipMemcpyAsync(c->pf.p, pf, pf_bytes, hipMemcpyHostToDevice, c->stream));
-    rc = run_device(c, p, c->pf.p, (int32_t*)c->out_cl1.p, (int32_t*)c->out_cl2.p, (int32_t*)c->out_j.p,
-                    (double*)c->out_corr.p, (double*)c->out_shift.p, st);
+    const clm_stack_io io{c->pf.p, (int32_t*)c->out_cl1.p, (int32_t*)c->out_cl2.p, (int32_t*)c->out_j.p,
+                          (double*)c->out_corr.p, (double*)c->out_shift.p};
+    rc = run_job(c, 1, p, &io, false, st);
     if (rc) return rc;
     if (P > 0) {
         if (cl1) HIPCHK(c, hipMemcpyAsync(cl1, c->out_cl1.p, P * 4, hipMemcpyDeviceToHost, c->stream));

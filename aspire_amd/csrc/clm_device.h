@@ -45,4 +45,38 @@ struct Geom {
     int NAT;      // ceil(NROWS / 32)
 };
 
+// Everything the kernels need about one stack (one cryo_clmatrix call).  A
+// launch serves one or more stacks of the same kernel variant; tiles and
+// list entries name their stack.
+struct StackDev {
+    Geom g;
+    const void* astk;         // [K][NAT] A-tiles of the operand type
+    const void* Bop;          // [K][2][Lp][Rp] operand planes
+    const float* nrm;         // [K] largest half-line norm per image
+    PairRec* recs;            // [P] pass-1 records
+    const float2* P32;        // [K][L][Rp]
+    const double2* P64;       // [K][L][R]
+    const float2* ph32;       // [NE+1][Rp], row NE zero
+    const double2* ph64;      // [S][R]
+    const double* shiftv;     // [S]
+    int32_t* cl1;             // outputs, [P] in equation order
+    int32_t* cl2;
+    int32_t* jidx;
+    double* corr;
+    double* shift1d;
+    float coef, sub;          // error bound: delta = coef*n1*n2 + sub*(n1+n2)
+    float tau_t;              // tie margin in t units (tau/2)
+    float tau;                // FP64 re-score margin (correlation units)
+    int fp64;                 // re-score near ties in FP64
+    int CPP;                  // waves per pair
+    long long pair_base;      // first flat pair index of this stack within its group
+    long long npairs;
+};
+
+// Pass-2 / fallback list entry: (stack, slot)
+struct SlotRef {
+    int32_t stack;
+    int32_t slot;
+};
+
 }  // namespace clm

@@ -124,24 +124,73 @@ __global__ void prep_kernel(const InT* __restrict__ pf, Geom g,
     if (isfinite(hf)) atomicMax(nrm_bits + k, __float_as_uint(hf));
 }
 
+// ------------------------------------------------------------- A-stacks
+// Image k's shift-expanded rays ("A-stack"): rows (e, l1) of P_k .* ph_e,
+// rounded once to the operand type, 32 rows per A-tile, each row stored as
+// CPR 16-byte chunks (re half, then im half).  Chunk c of row r is stored at
+// position (c + r) mod CPR, so the MFMA fragment reads (32 rows at the same
+// logical chunk) hit distinct LDS banks without row padding.  Rows past
+// the last shift are zero.  An A-tile is one contiguous block that the pair
+// kernel copies to LDS with direct global-to-LDS loads.
+template <typename OpT, int NK> struct ATile {
+    static constexpr int Rp = NK * Op<OpT>::KSTEP;
+    static constexpr int CPR = 2 * Rp * (int)sizeof(OpT) / 16;  // chunks per row
+    static constexpr int EPC = 16 / (int)sizeof(OpT);            // elements per chunk
+    static constexpr int NCH = 32 * CPR;                         // chunks per tile
+    static constexpr int BYTES = NCH * 16;
+    __device__ static inline int phys(int row, int c) { return (c + row) % CPR; }
+};
+
+template <typename OpT, int NK, bool SYM>
+__global__ void astack_kernel(Geom g, const float2* __restrict__ P32, const float2* __restrict__ ph32,
+                              int4* __restrict__ astk) {
+    using AT = ATile<OpT, NK>;
+    constexpr int Rp = AT::Rp, CPR = AT::CPR, EPC = AT::EPC;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)g.K * g.NAT * AT::NCH) return;
+    const int pch = (int)(q % CPR);
+    long r = q / CPR;
+    const int row = (int)(r % 32);
+    r /= 32;
+    const int at = (int)(r % g.NAT);
+    const int k = (int)(r / g.NAT);
+    const int c = (pch - row % CPR + CPR) % CPR;   // logical chunk stored here
+    const bool imag = c >= CPR / 2;
+    const int k0 = (c % (CPR / 2)) * EPC;
+    const int gr = at * 32 + row;
+    OpT v[EPC];
+    if (gr < g.NROWS) {
+        const int e = gr / g.L, l1 = gr - e * g.L;
+        const float sg = (!SYM && e >= g.S) ? -1.0f : 1.0f;
+        const float2* p = P32 + ((size_t)k * g.L + l1) * Rp + k0;
+        const float2* ph = ph32 + (size_t)e * Rp + k0;
+#pragma unroll
+        for (int u = 0; u < EPC; u++) {
+            const float2 pv = p[u], qv = ph[u];
+            const float re = fmaf(pv.x, qv.x, -pv.y * qv.y);
+            const float im = fmaf(pv.x, qv.y, pv.y * qv.x);
+            v[u] = (OpT)(imag ? sg * im : re);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < EPC; u++) v[u] = (OpT)0.0f;
+    }
+    int4 out;
+    __builtin_memcpy(&out, v, 16);
+    astk[q] = out;
+}
+
 // -------------------------------------------------------------- pair kernel
 struct PairArgs {
-    Geom g;
-    const float2* P32;     // [K][L][Rp]
-    const void* Bop;       // [K][2][Lp][Rp] operand planes
-    const float2* ph32;    // [NE + 1][Rp] (cos, sin) of the evaluation shifts; row NE zero
-    const float* nrm;      // [K]
-    const int4* tiles;     // {k1, k2_first | list offset, count, slot | -1}
-    int ntiles;
+    const StackDev* stacks;
+    const int4* tiles;     // {k1, k2_first | list offset, count | stack << 4, slot | -1}
+    int ntiles;            // tile count, or
+    const int* ntiles_dev; // (when set) tile count in device memory
     int* counter;          // next tile (persistent work queue)
     const int4* plist;     // explicit pairs {k2, slot, u, 0}
-    PairRec* recs;         // indexed by slot
     EmitRec* erecs;        // indexed by u (second pass)
-    int CPP;               // waves per pair (column chunks)
-    float coef;            // delta = coef * n1 * n2 + sub * (n1 + n2)
-    float sub;
-    float tau_t;           // tie margin in t units
-    int stage;             // 1: image k1's rays and the phase table live in LDS
+    int CPP;               // waves per pair (column chunks); same for all stacks
+    int lag;               // 1: one wave per SIMD runs one A-tile behind
 };
 
 constexpr int WCAP = 64;   // per-wave list of candidate cells (one per lane)
@@ -164,20 +213,16 @@ __device__ inline float wave_max_dpp(float v) {
     return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
 }
 
-// Dynamic LDS bytes of pair_kernel for a launch shape.
-__host__ __device__ inline size_t pair_lds_bytes(int Rp, int osz, int W, int NE, int L, int stage) {
-    size_t b = 64 * (size_t)(Rp * 2 * osz + 16) + 384 + (size_t)W * WCAP * 8;
-    if (stage) b += (size_t)(NE + 1 + L) * Rp * sizeof(float2);
-    return b;
-}
+// Dynamic LDS bytes of pair_kernel (the A-tile ring is static).
+__host__ __device__ inline size_t pair_lds_bytes(int W) { return 384 + (size_t)W * WCAP * 8; }
 
 // Persistent all-pairs kernel.  A workgroup takes a tile (one image k1 and up
-// to W/CPP partner images k2), stages k1's rays in LDS, and sweeps the rows
-// (e, l1) of k1's shift-expanded rays in A-tiles of 32 rows.  Each A-tile is
-// generated once in LDS (rays x phase, rounded to the operand type) and
-// multiplied by every wave against its k2's columns, which stay in
-// registers.  The epilogue reduces each lane's 32x32 output tile to one
-// "cell" maximum of t = X + |Y|.
+// to W/CPP partner images k2).  k1's A-tiles (32 rows (e, l1) of its
+// shift-expanded rays each, precomputed by astack_kernel) stream through two
+// LDS buffers, prefetched one tile ahead through registers; each wave
+// multiplies every A-tile against its k2's columns, which stay in
+// registers.  The epilogue reduces each lane's 32x32 output tile to
+// one "cell" maximum of t = X + |Y|.
 //   EMIT = false (pass 1): each wave keeps every cell that comes within the
 //     certified margin of the pair's running maximum; at the end the cells
 //     within the margin of the pair's final maximum go to the record.
@@ -185,31 +230,39 @@ __host__ __device__ inline size_t pair_lds_bytes(int Rp, int osz, int W, int NE,
 //     from pass 1 is listed.
 template <typename OpT, int CT, int NK, bool SYM, bool EMIT>
 __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
-    extern __shared__ __align__(16) char smem[];
     using O = Op<OpT>;
     using frag = typename O::frag;
-    const Geom g = a.g;
+    using AT = ATile<OpT, NK>;
+    constexpr int Rp = AT::Rp, CPR = AT::CPR, NCH = AT::NCH;
+    constexpr int NPT = (NCH + 255) / 256;    // chunks per thread (W >= 4)
+    __shared__ __align__(16) int4 sa0[NCH];
+    __shared__ __align__(16) int4 sa1[NCH];
+    extern __shared__ __align__(16) char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = blockDim.x >> 6;
     const int h = lane >> 5, lc = lane & 31;
-    constexpr int Rp = NK * O::KSTEP;                   // padded ray length (== g.Rp)
-    constexpr int RS = Rp * 2 * (int)sizeof(OpT) + 16;  // LDS row stride (bytes)
-    char* abuf0 = smem;
-    char* abuf1 = smem + 32 * RS;
-    int* s_i = (int*)(smem + 64 * RS);        // [0] tile, [1..] per-pair counters
+    int* s_i = (int*)smem;                    // [0] tile, [1..] per-pair counters
     float* s_pm = (float*)(s_i + 32);         // per-wave running maxima
     int* s_unc = (int*)(s_pm + 32);           // per-pair flags
     float2* s_list = (float2*)(s_unc + 32);   // [W][WCAP] (value, id bits)
-    float2* s_ph = s_list + W * WCAP;         // [NE + 1][Rp], row NE zero
-    float2* s_p1 = s_ph + (size_t)(g.NE + 1) * Rp;  // [L][Rp]
     float2* wlist = s_list + wave * WCAP;
     const int pi = wave / a.CPP, chunk = wave % a.CPP;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
 
-    if (a.stage) {
-        const int n4 = (g.NE + 1) * Rp / 2;
-        const float4* src = (const float4*)a.ph32;
-        for (int i = tid; i < n4; i += blockDim.x) ((float4*)s_ph)[i] = src[i];
+    // Which waves lag: the second wave (by index) on each SIMD, from the SIMD
+    // id in HW_REG_HW_ID (bits 5:4), shared through LDS.
+    bool lag_wave = false;
+    if (a.lag) {
+        const unsigned hwid = __builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 4);
+        if (lane == 0) s_unc[wave] = (int)((hwid >> 4) & 3);
+        __syncthreads();
+        const int my = s_unc[wv];
+        for (int w = 0; w < wv; w++)
+            if (s_unc[w] == my) lag_wave = true;
+        __syncthreads();
     }
+    const bool lag = a.lag && lag_wave;
+    const int ntiles = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
 
     for (;;) {
         if (tid == 0) s_i[0] = atomicAdd(a.counter, 1);
@@ -217,24 +270,43 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         if (tid < 32) { s_unc[tid] = 0; s_pm[tid] = -INFINITY; }
         __syncthreads();
         const int t = s_i[0];
-        if (t >= a.ntiles) break;
+        if (t >= ntiles) break;
         const int4 tile = a.tiles[t];
         const int k1 = tile.x;
-        const bool active = pi < tile.z;
+        const int cnt_t = tile.z & 15;
+        const StackDev& sd = a.stacks[tile.z >> 4];
+        const Geom g = sd.g;
+        const bool active = pi < cnt_t;
         int k2 = 0, slot = 0, uidx = 0;
         if (active) {
             if (tile.w >= 0) { k2 = tile.y + pi; slot = tile.w + pi; }
             else { int4 pe = a.plist[tile.y + pi]; k2 = pe.x; slot = pe.y; uidx = pe.z; }
         }
-        if (a.stage) {
-            const int n4 = g.L * Rp / 2;
-            const float4* src = (const float4*)(a.P32 + (size_t)k1 * g.L * Rp);
-            for (int i = tid; i < n4; i += blockDim.x) ((float4*)s_p1)[i] = src[i];
-        }
+        // A-tiles of image k1 stream through two LDS buffers: each thread
+        // loads its 16-byte chunks of tile at+1 into registers one step
+        // ahead and writes them to LDS at the start of the next step.
+        const int4* asrc = (const int4*)((const char*)sd.astk + (size_t)k1 * g.NAT * AT::BYTES);
+        int4 pre[NPT];
+        auto load_pre = [&](int at) {
+            const int4* src = asrc + (size_t)at * NCH;
+#pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                const int q = tid + j * (int)blockDim.x;
+                if (q < NCH) pre[j] = src[q];
+            }
+        };
+        auto store_pre = [&](int4* buf) {
+#pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                const int q = tid + j * (int)blockDim.x;
+                if (q < NCH) buf[q] = pre[j];
+            }
+        };
+        load_pre(0);
         // ---- B operand (image k2, this wave's CT column tiles) in registers
         frag Bre[CT][NK], Bim[CT][NK];
         {
-            const OpT* B = (const OpT*)a.Bop;
+            const OpT* B = (const OpT*)sd.Bop;
             const size_t plane = (size_t)g.Lp * Rp;
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
@@ -253,210 +325,147 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
                 }
             }
         }
-        const float n1 = a.nrm[k1], n2 = a.nrm[k2];
-        const float margin = 2.0f * (a.coef * n1 * n2 + a.sub * (n1 + n2)) + a.tau_t;
+        const float n1 = sd.nrm[k1], n2 = sd.nrm[k2];
+        const float margin = 2.0f * (sd.coef * n1 * n2 + sd.sub * (n1 + n2)) + sd.tau_t;
         float thr = -INFINITY;
-        if (EMIT && active) thr = a.recs[slot].T;
+        if (EMIT && active) thr = sd.recs[slot].T;
         float b = -INFINITY;   // lane's running cell maximum
         float pm = -INFINITY;  // pair's running maximum (all waves, previous A-tile)
         int wc = 0;            // entries in this wave's cell list
         bool lost = false;     // a cell within the margin could not be kept
 
-        // ---- A-tile generation: rows (e,l1) of image k1 times the shift
-        // phase, rounded once to the operand type.  One item = 4 complex
-        // values of one row (two float4 of rays, two of phases); consecutive
-        // threads take consecutive items.  (g_e0, g_l10) is the (e, l1) of
-        // row 0 of the next tile to generate, advanced by 32 rows per tile.
-        constexpr int QC = Rp / 4;        // items per row
-        constexpr int NC = 32 * QC;       // items per tile
-        int g_e0 = 0, g_l10 = 0;
-        const float2* p1_glb = a.P32 + (size_t)k1 * g.L * Rp;
-        auto gen = [&](int at, char* buf) {
-            for (int it = tid; it < NC; it += blockDim.x) {
-                const int row = it / QC, q = it - row * QC;
-                int e = g_e0, l1 = g_l10 + row;
-                if (g.L >= 32) {
-                    if (l1 >= g.L) { l1 -= g.L; e++; }
-                } else {
-                    e += l1 / g.L;
-                    l1 -= (l1 / g.L) * g.L;
-                }
-                // rows past the last shift read the zero phase row NE, so
-                // padding rows of the last tile come out as exact zeros
-                const int ep = min(e, g.NE);
-                float4 p0, p1v, q0, q1;
-                if (a.stage) {
-                    const float4* pp = (const float4*)(s_p1 + l1 * Rp + 4 * q);
-                    const float4* qp = (const float4*)(s_ph + ep * Rp + 4 * q);
-                    p0 = pp[0]; p1v = pp[1]; q0 = qp[0]; q1 = qp[1];
-                } else {
-                    const float4* pp = (const float4*)(p1_glb + (size_t)l1 * Rp + 4 * q);
-                    const float4* qp = (const float4*)(a.ph32 + (size_t)ep * Rp + 4 * q);
-                    p0 = pp[0]; p1v = pp[1]; q0 = qp[0]; q1 = qp[1];
-                }
-                float re[4], im[4];
-                re[0] = fmaf(p0.x, q0.x, -p0.y * q0.y);
-                im[0] = fmaf(p0.x, q0.y, p0.y * q0.x);
-                re[1] = fmaf(p0.z, q0.z, -p0.w * q0.w);
-                im[1] = fmaf(p0.z, q0.w, p0.w * q0.z);
-                re[2] = fmaf(p1v.x, q1.x, -p1v.y * q1.y);
-                im[2] = fmaf(p1v.x, q1.y, p1v.y * q1.x);
-                re[3] = fmaf(p1v.z, q1.z, -p1v.w * q1.w);
-                im[3] = fmaf(p1v.z, q1.w, p1v.w * q1.z);
-                if (!SYM && e >= g.S) {
+        float m[CT];
+        f32x16 X[CT], Y[CT];
+        // best of both maps over a cell: t = X + |Y|; pass 2 also lists
+        // every candidate of tile tat at or above the threshold
+        auto epi = [&](int ct, int tat) {
+            float mv = -INFINITY;
 #pragma unroll
-                    for (int u = 0; u < 4; u++) im[u] = -im[u];
-                }
-                OpT* dre = (OpT*)(buf + row * RS) + 4 * q;
-                OpT* dim = dre + Rp;
-                if constexpr (O::KSTEP == 16) {
-                    typedef OpT op4 __attribute__((ext_vector_type(4)));
-                    op4 vr, vi;
+            for (int v = 0; v < 16; v++)
+                mv = fmaxf(mv, SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v]);
+            m[ct] = mv;
+            if constexpr (EMIT) {
+                if (mv >= thr) {
+                    const int col = (chunk * CT + ct) * 32 + lc;
 #pragma unroll
-                    for (int u = 0; u < 4; u++) { vr[u] = (OpT)re[u]; vi[u] = (OpT)im[u]; }
-                    *(op4*)dre = vr;
-                    *(op4*)dim = vi;
-                } else {
-                    *(float4*)dre = make_float4(re[0], re[1], re[2], re[3]);
-                    *(float4*)dim = make_float4(im[0], im[1], im[2], im[3]);
-                }
-            }
-            g_l10 += 32;
-            while (g_l10 >= g.L) { g_l10 -= g.L; g_e0++; }
-        };
-
-        if (a.stage) __syncthreads();  // k1's rays staged
-        gen(0, abuf0);
-        __syncthreads();
-        for (int at = 0; at < g.NAT; at++) {
-            char* cur = (at & 1) ? abuf1 : abuf0;
-            char* nxt = (at & 1) ? abuf0 : abuf1;
-            const char* arow = cur + lc * RS;
-            float m[CT];
-            f32x16 X[CT], Y[CT];
-            frag fa[O::KSTEP == 16 ? NK : 1], fb[O::KSTEP == 16 ? NK : 1];
-            // best of both maps over a cell: t = X + |Y|; pass 2 also lists
-            // every candidate at or above the threshold
-            auto epi = [&](int ct) {
-                float mv = -INFINITY;
-#pragma unroll
-                for (int v = 0; v < 16; v++)
-                    mv = fmaxf(mv, SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v]);
-                m[ct] = mv;
-                if constexpr (EMIT) {
-                    if (mv >= thr) {
-                        const int col = (chunk * CT + ct) * 32 + lc;
-#pragma unroll
-                        for (int v = 0; v < 16; v++) {
-                            const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
-                            const int row = at * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-                            if (tv >= thr && row < g.NROWS && col < g.L) {
-                                const int pos = atomicAdd(&s_i[1 + pi], 1);
-                                if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
-                            }
+                    for (int v = 0; v < 16; v++) {
+                        const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
+                        const int row = tat * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+                        if (tv >= thr && row < g.NROWS && col < g.L) {
+                            const int pos = atomicAdd(&s_i[1 + pi], 1);
+                            if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
                         }
                     }
                 }
-            };
-            auto mma16 = [&](int ct) {
-                f32x16 z = {0.f};
-                X[ct] = O::mma(fa[0], Bre[ct][0], z);
-                Y[ct] = O::mma(fb[0], Bim[ct][0], z);
+            }
+        };
+        // pass 1: keep the cells of tile tat that come within the margin
+        // of the pair's running maximum; publish this wave's maximum
+        auto keep_cells = [&](int tat) {
+            if constexpr (!EMIT) {
+                float mm = m[0];
 #pragma unroll
-                for (int ks = 1; ks < NK; ks++) {
-                    X[ct] = O::mma(fa[ks], Bre[ct][ks], X[ct]);
-                    Y[ct] = O::mma(fb[ks], Bim[ct][ks], Y[ct]);
-                }
-            };
-            if constexpr (O::KSTEP == 16) {
-                // A fragments of this tile, then the first column tile's MFMAs,
-                // then the next tile's generation while they run
-                if (active) {
-#pragma unroll
-                    for (int ks = 0; ks < NK; ks++) {
-                        fa[ks] = *(const frag*)((const OpT*)arow + 16 * ks + 8 * h);
-                        fb[ks] = *(const frag*)((const OpT*)arow + Rp + 16 * ks + 8 * h);
-                    }
-                    mma16(0);
-                }
-                if (at + 1 < g.NAT) gen(at + 1, nxt);
-                if (active) {
+                for (int ct = 1; ct < CT; ct++) mm = fmaxf(mm, m[ct]);
+                b = fmaxf(b, mm);
+                // wave-uniform threshold from the running maxima seen so
+                // far (all <= the pair's final maximum)
+                const float wm = wave_max_dpp(b);
+                if (lane == 0) s_pm[wave] = wm;
+                const float th = fmaxf(wm, pm) - margin;
+                if (__ballot(mm >= th && mm > -INFINITY)) {
+                    // rare once the pair's maximum has been seen
 #pragma unroll
                     for (int ct = 0; ct < CT; ct++) {
-                        if (ct + 1 < CT) mma16(ct + 1);
-                        epi(ct);
-                    }
-                }
-            } else {
-                if (at + 1 < g.NAT) gen(at + 1, nxt);
-                if (active) {
-                    // FP32 operands: one k-step at a time over every column tile
-#pragma unroll
-                    for (int ks = 0; ks < NK; ks++) {
-                        const frag a0 = ((const float*)arow)[h * (Rp / 2) + ks];
-                        const frag b0 = ((const float*)arow)[Rp + h * (Rp / 2) + ks];
-#pragma unroll
-                        for (int ct = 0; ct < CT; ct++) {
-                            if (ks == 0) {
-                                f32x16 z = {0.f};
-                                X[ct] = O::mma(a0, Bre[ct][0], z);
-                                Y[ct] = O::mma(b0, Bim[ct][0], z);
-                            } else {
-                                X[ct] = O::mma(a0, Bre[ct][ks], X[ct]);
-                                Y[ct] = O::mma(b0, Bim[ct][ks], Y[ct]);
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int ct = 0; ct < CT; ct++) epi(ct);
-                }
-            }
-            if (active) {
-                if constexpr (!EMIT) {
-                    float mm = m[0];
-#pragma unroll
-                    for (int ct = 1; ct < CT; ct++) mm = fmaxf(mm, m[ct]);
-                    b = fmaxf(b, mm);
-                    // wave-uniform threshold from the running maxima seen so
-                    // far (all <= the pair's final maximum)
-                    const float wm = wave_max_dpp(b);
-                    if (lane == 0) s_pm[wave] = wm;
-                    float th = fmaxf(wm, pm) - margin;
-                    if (__ballot(mm >= th && mm > -INFINITY)) {
-                        // rare once the pair's maximum has been seen
-#pragma unroll
-                        for (int ct = 0; ct < CT; ct++) {
-                            bool e = (m[ct] >= th) && (m[ct] > -INFINITY);
-                            unsigned long long mask = __ballot(e);
-                            if (!mask) continue;
-                            int cnt = __popcll(mask);
-                            if (wc + cnt > WCAP) {
-                                // compact: drop entries below the current threshold
-                                const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
-                                const bool keep = lane < wc && ent.x >= th;
-                                const unsigned long long km = __ballot(keep);
-                                __builtin_amdgcn_wave_barrier();
-                                if (keep) wlist[__popcll(km & lanemask_lt())] = ent;
-                                wc = __popcll(km);
-                                __builtin_amdgcn_wave_barrier();
-                            }
-                            if (e) {
-                                const int pos = wc + __popcll(mask & lanemask_lt());
-                                const uint32_t id = ((uint32_t)(at * g.NCT + chunk * CT + ct) << 6) | (uint32_t)lane;
-                                if (pos < WCAP) wlist[pos] = make_float2(m[ct], __uint_as_float(id));
-                            }
-                            if (wc + cnt > WCAP) lost = true;
-                            wc = min(wc + cnt, WCAP);
+                        const bool e = (m[ct] >= th) && (m[ct] > -INFINITY);
+                        const unsigned long long mask = __ballot(e);
+                        if (!mask) continue;
+                        const int cnt = __popcll(mask);
+                        if (wc + cnt > WCAP) {
+                            // compact: drop entries below the current threshold
+                            const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
+                            const bool keep = lane < wc && ent.x >= th;
+                            const unsigned long long km = __ballot(keep);
+                            __builtin_amdgcn_wave_barrier();
+                            if (keep) wlist[__popcll(km & lanemask_lt())] = ent;
+                            wc = __popcll(km);
                             __builtin_amdgcn_wave_barrier();
                         }
+                        if (e) {
+                            const int pos = wc + __popcll(mask & lanemask_lt());
+                            const uint32_t id = ((uint32_t)(tat * g.NCT + chunk * CT + ct) << 6) | (uint32_t)lane;
+                            if (pos < WCAP) wlist[pos] = make_float2(m[ct], __uint_as_float(id));
+                        }
+                        if (wc + cnt > WCAP) lost = true;
+                        wc = min(wc + cnt, WCAP);
+                        __builtin_amdgcn_wave_barrier();
                     }
                 }
+            }
+        };
+        // one A-tile: write tile at+1 (loaded during the previous step) to
+        // the other buffer and start loading tile at+2; reduce (lagging
+        // waves: the previous tile), multiply, reduce (leading waves: this
+        // tile)
+        auto step = [&](int at, const int4* cur, int4* nxt) {
+            if (at + 1 < g.NAT) {
+                store_pre(nxt);
+                if (at + 2 < g.NAT) load_pre(at + 2);
+            }
+            if (lag && active && at > 0) {
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++) epi(ct, at - 1);
+                keep_cells(at - 1);
+            }
+            if (active) {
+                const char* arow = (const char*)cur + lc * CPR * 16;
+#pragma unroll
+                for (int ks = 0; ks < NK; ks++) {
+                    frag a0, b0;
+                    if constexpr (O::KSTEP == 16) {
+                        a0 = *(const frag*)(arow + AT::phys(lc, 2 * ks + h) * 16);
+                        b0 = *(const frag*)(arow + AT::phys(lc, CPR / 2 + 2 * ks + h) * 16);
+                    } else {
+                        const int kk = h * (Rp / 2) + ks;
+                        a0 = *(const float*)(arow + AT::phys(lc, kk / 4) * 16 + (kk % 4) * 4);
+                        b0 = *(const float*)(arow + AT::phys(lc, CPR / 2 + kk / 4) * 16 + (kk % 4) * 4);
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < CT; ct++) {
+                        if (ks == 0) {
+                            f32x16 z = {0.f};
+                            X[ct] = O::mma(a0, Bre[ct][0], z);
+                            Y[ct] = O::mma(b0, Bim[ct][0], z);
+                        } else {
+                            X[ct] = O::mma(a0, Bre[ct][ks], X[ct]);
+                            Y[ct] = O::mma(b0, Bim[ct][ks], Y[ct]);
+                        }
+                    }
+                }
+            }
+            if (!lag && active) {
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++) epi(ct, at);
+                keep_cells(at);
             }
             __syncthreads();
             if (!EMIT && active) {
                 for (int c = 0; c < a.CPP; c++) pm = fmaxf(pm, s_pm[pi * a.CPP + c]);
             }
+        };
+
+        store_pre(sa0);
+        if (g.NAT > 1) load_pre(1);
+        __syncthreads();
+        for (int at = 0; at < g.NAT; at += 2) {
+            step(at, sa0, sa1);
+            if (at + 1 < g.NAT) step(at + 1, sa1, sa0);
         }
+        if (lag && active) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) epi(ct, g.NAT - 1);
+            keep_cells(g.NAT - 1);
+        }
+        __syncthreads();
 
         // ---- per-pair finalisation
         if constexpr (!EMIT) {
@@ -468,7 +477,7 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
                 const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
                 if (lane < wc && ent.x >= T) {
                     const int pos = atomicAdd(&s_i[1 + pi], 1);
-                    if (pos < MAXC) a.recs[slot].cell[pos] = __float_as_uint(ent.y);
+                    if (pos < MAXC) sd.recs[slot].cell[pos] = __float_as_uint(ent.y);
                 }
                 if (lost) atomicOr(&s_unc[pi], 1);
             }
@@ -477,7 +486,7 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
                 float M = -INFINITY;
                 for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
                 const int n = s_i[1 + pi];
-                PairRec* r = a.recs + slot;
+                PairRec* r = sd.recs + slot;
                 r->M = M;
                 r->T = M - margin;
                 r->k1 = k1;
@@ -500,27 +509,21 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
 
 // ------------------------------------------------------------ re-scoring
 struct RescoreArgs {
-    Geom g;
-    const float2* P32;        // [K][L][Rp]
-    const double2* P64;       // [K][L][R] (null for plain FP32)
-    const float2* ph32;       // [NE][Rp]
-    const double2* ph64;      // [S][R] phases of the reference grid
-    const double* shiftv;     // [S] s_j
-    const PairRec* recs;      // pass 1 records (by slot)
-    const EmitRec* erecs;     // pass 2 candidate lists (by u)
-    const int* ulist;         // pass 2: u -> slot
-    int n_items;              // pairs (pass 1) or uncertain pairs (pass 2)
-    int mode;                 // 0: cells of pass 1, 1: emitted candidates
-    int fp64;                 // re-score near ties in FP64
-    float tau;                // near-tie margin (correlation units)
-    int* unc_count;           // pass 1: uncertain pairs appended here
-    int* unc_list;
-    int* fb_count;            // pass 2: overflowing pairs appended here
-    int* fb_list;
+    const StackDev* stacks;   // the whole table (SlotRef.stack indexes it)
+    int stack0, nstacks;      // mode 0: stacks [stack0, stack0 + nstacks)
+    int mode;                 // 0: every pair of those stacks (pass 1), 1: pass-2 list
+    long long n_items;        // mode 0: total pairs of all stacks
+    const int* n_items_dev;   // mode 1: number of list entries (device)
+    int item_cap;             // mode 1: entries beyond this were not given a pass 2
+    const SlotRef* ulist;     // mode 1: u -> (stack, slot)
+    const EmitRec* erecs;     // mode 1: candidates of entry u
+    int* unc_count;           // mode 0: uncertain pairs appended here
+    SlotRef* unc_list;
+    int* fb_count;            // both: pairs for the FP64 exhaustive search
+    SlotRef* fb_list;
     unsigned long long* n64;  // FP64 re-scored candidates (stats)
     unsigned long long* ncell;  // cells re-scored (stats)
     unsigned long long* nemit;  // second-pass candidates (stats)
-    int32_t* cl1; int32_t* cl2; int32_t* jidx; double* corr; double* shift1d;
 };
 
 // Order key of the reference's tie rule (shift index first, then the column
@@ -550,12 +553,21 @@ __device__ inline double entry64(const double2* P1, const double2* P2, const dou
     return 2.0 * acc;
 }
 
+// Stack of flat pair index `item` (stacks sorted by pair_base).
+__device__ inline int find_stack(const StackDev* st, int n, long long item) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (st[mid].pair_base <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Decode candidate q of a pair: pass-1 cells expand to 16 rows each,
 // pass-2 lists hold (row, col) directly.  row = -1 marks padding.
-__device__ inline void cand_rc(const RescoreArgs& a, const PairRec& rec, const EmitRec* er,
+__device__ inline void cand_rc(const Geom& g, int mode, const PairRec& rec, const EmitRec* er,
                                int q, int& row, int& col) {
-    const Geom& g = a.g;
-    if (a.mode == 0) {
+    if (mode == 0) {
         const uint32_t c = rec.cell[q >> 4];
         const int cl = c & 63, cid = c >> 6;
         const int at = cid / g.NCT, ct = cid % g.NCT;
@@ -570,33 +582,33 @@ __device__ inline void cand_rc(const RescoreArgs& a, const PairRec& rec, const E
     if (row >= g.NROWS || col >= g.L) row = -1;
 }
 
-__device__ inline double eval64(const RescoreArgs& a, int k1, int k2, long long key) {
-    const Geom& g = a.g;
+__device__ inline double eval64(const StackDev& sd, int k1, int k2, long long key) {
+    const Geom& g = sd.g;
     const int l1 = (int)(key % g.L);
     const int l2p = (int)((key / g.L) % (2 * g.L));
     const int j = (int)(key / ((long long)g.L * 2 * g.L));
-    const double2* P1 = a.P64 + ((size_t)k1 * g.L + l1) * g.R;
-    const double2* P2 = a.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
-    return entry64(P1, P2, a.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
+    const double2* P1 = sd.P64 + ((size_t)k1 * g.L + l1) * g.R;
+    const double2* P2 = sd.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
+    return entry64(P1, P2, sd.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
 }
 
-__device__ inline void write_result(const RescoreArgs& a, int slot, double v, long long key) {
-    const Geom& g = a.g;
+__device__ inline void write_result(const StackDev& sd, int slot, double v, long long key) {
+    const Geom& g = sd.g;
     if (v > 0.0) {
         const int l1 = (int)(key % g.L);
         const int l2p = (int)((key / g.L) % (2 * g.L));
         const int j = (int)(key / ((long long)g.L * 2 * g.L));
-        if (a.cl1) a.cl1[slot] = l1 + 1;
-        if (a.cl2) a.cl2[slot] = l2p + 1;
-        if (a.jidx) a.jidx[slot] = j;
-        if (a.corr) a.corr[slot] = v;
-        if (a.shift1d) a.shift1d[slot] = a.shiftv[j];
+        if (sd.cl1) sd.cl1[slot] = l1 + 1;
+        if (sd.cl2) sd.cl2[slot] = l2p + 1;
+        if (sd.jidx) sd.jidx[slot] = j;
+        if (sd.corr) sd.corr[slot] = v;
+        if (sd.shift1d) sd.shift1d[slot] = sd.shiftv[j];
     } else {
-        if (a.cl1) a.cl1[slot] = 0;
-        if (a.cl2) a.cl2[slot] = 0;
-        if (a.jidx) a.jidx[slot] = -1;
-        if (a.corr) a.corr[slot] = 0.0;
-        if (a.shift1d) a.shift1d[slot] = 0.0;
+        if (sd.cl1) sd.cl1[slot] = 0;
+        if (sd.cl2) sd.cl2[slot] = 0;
+        if (sd.jidx) sd.jidx[slot] = -1;
+        if (sd.corr) sd.corr[slot] = 0.0;
+        if (sd.shift1d) sd.shift1d[slot] = 0.0;
     }
 }
 
@@ -611,29 +623,38 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = blockDim.x >> 6;
     float* sval = s_val[wave];
-    const Geom& g = a.g;
     const int q = lane & 3, rl = lane >> 2;  // k-quarter, row within the step
-    const int kq = g.Rp / 4;                 // multiple of 4
-    for (int item = blockIdx.x * W + wave; item < a.n_items; item += gridDim.x * W) {
-        const int slot = a.mode == 0 ? item : a.ulist[item];
-        const PairRec rec = a.recs[slot];
+    const long long n_items = a.mode == 0 ? a.n_items : (long long)min(*a.n_items_dev, a.item_cap);
+    for (long long item = (long long)blockIdx.x * W + wave; item < n_items; item += (long long)gridDim.x * W) {
+        int stk, slot;
+        if (a.mode == 0) {
+            stk = a.stack0 + (a.nstacks > 1 ? find_stack(a.stacks + a.stack0, a.nstacks, item) : 0);
+            slot = (int)(item - a.stacks[stk].pair_base);
+        } else {
+            stk = a.ulist[item].stack;
+            slot = a.ulist[item].slot;
+        }
+        const StackDev& sd = a.stacks[stk];
+        const Geom& g = sd.g;
+        const int kq = g.Rp / 4;  // multiple of 4
+        const PairRec rec = sd.recs[slot];
         const int k1 = rec.k1, k2 = rec.k2;
         const EmitRec* er = a.mode ? a.erecs + item : nullptr;
         int nrows;
         if (a.mode == 0) {
             if (rec.flags & F_EMPTY) {
-                if (lane == 0) write_result(a, slot, 0.0, 0);
+                if (lane == 0) write_result(sd, slot, 0.0, 0);
                 continue;
             }
             if (rec.flags & (F_UNCERTAIN | F_OVERFLOW)) {
-                if (lane == 0) { const int u = atomicAdd(a.unc_count, 1); a.unc_list[u] = slot; }
+                if (lane == 0) { const int u = atomicAdd(a.unc_count, 1); a.unc_list[u] = SlotRef{stk, slot}; }
                 continue;
             }
             nrows = 16 * rec.ncells;
             if (lane == 0 && a.ncell) atomicAdd(a.ncell, (unsigned long long)rec.ncells);
         } else {
             if (er->overflow) {
-                if (lane == 0) { const int u = atomicAdd(a.fb_count, 1); a.fb_list[u] = slot; }
+                if (lane == 0) { const int u = atomicAdd(a.fb_count, 1); a.fb_list[u] = SlotRef{stk, slot}; }
                 continue;
             }
             nrows = er->n;
@@ -645,15 +666,15 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         for (int base = 0; base < nrows; base += 16) {
             const int r = base + rl;
             int row = -1, col = 0;
-            if (r < nrows) cand_rc(a, rec, er, r, row, col);
+            if (r < nrows) cand_rc(g, a.mode, rec, er, r, row, col);
             float X = 0.f, Y = 0.f;
             int e = 0, l1 = 0;
             if (row >= 0) {
                 e = row / g.L;
                 l1 = row - e * g.L;
-                const float4* p = (const float4*)(a.P32 + ((size_t)k1 * g.L + l1) * g.Rp + q * kq);
-                const float4* bb = (const float4*)(a.P32 + ((size_t)k2 * g.L + col) * g.Rp + q * kq);
-                const float4* ph = (const float4*)(a.ph32 + (size_t)e * g.Rp + q * kq);
+                const float4* p = (const float4*)(sd.P32 + ((size_t)k1 * g.L + l1) * g.Rp + q * kq);
+                const float4* bb = (const float4*)(sd.P32 + ((size_t)k2 * g.L + col) * g.Rp + q * kq);
+                const float4* ph = (const float4*)(sd.ph32 + (size_t)e * g.Rp + q * kq);
                 for (int i = 0; i < kq / 2; i++) {
                     const float4 pv = p[i], qv = ph[i], b4 = bb[i];
                     const float ar0 = fmaf(pv.x, qv.x, -pv.y * qv.y);
@@ -698,10 +719,10 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         }
         double fv = (double)bv;
         long long fk = bk;
-        if (a.fp64 && bv > -INFINITY) {
+        if (sd.fp64 && bv > -INFINITY) {
             __builtin_amdgcn_wave_barrier();
             // ---- FP64 re-score of every candidate within tau of the FP32 best
-            const float lim = bv - a.tau;
+            const float lim = bv - sd.tau;
             double b64 = -INFINITY;
             long long k64 = 0x7fffffffffffffffLL;
             int n64 = 0;
@@ -710,12 +731,12 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
                 const int iv = base + lane;
                 if (iv < nv && sval[iv] >= lim) {
                     int row, col;
-                    cand_rc(a, rec, er, iv >> 1, row, col);
+                    cand_rc(g, a.mode, rec, er, iv >> 1, row, col);
                     const int e = row / g.L, l1 = row - e * g.L;
                     long long key;
                     if (g.sym) key = (iv & 1) ? tie_key(g.S - 1 - e, g.L + col, l1, g.L) : tie_key(e, col, l1, g.L);
                     else key = e < g.S ? tie_key(e, col, l1, g.L) : tie_key(e - g.S, g.L + col, l1, g.L);
-                    const double v = eval64(a, k1, k2, key);
+                    const double v = eval64(sd, k1, k2, key);
                     n64++;
                     if (!isnan(v) && better64(v, key, b64, k64)) { b64 = v; k64 = key; }
                 }
@@ -731,8 +752,29 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             fv = b64;
             fk = k64;
         }
-        if (lane == 0) write_result(a, slot, fv, fk);
+        if (lane == 0) write_result(sd, slot, fv, fk);
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Pass-2 tiles from the uncertain list, on the device: one pair per tile
+// (entry u -> tile u and explicit pair u).  Entries past `cap` have no
+// pass-2 buffers and go straight to the FP64 exhaustive list.
+__global__ void build_pass2_kernel(const StackDev* stacks, const int* n_unc, const SlotRef* ulist, int cap,
+                                   int4* tiles, int4* plist, int* ntiles, int* fb_count, SlotRef* fb_list) {
+    const int n = *n_unc;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *ntiles = min(n, cap);
+    for (int u = i; u < n; u += gridDim.x * blockDim.x) {
+        const SlotRef r = ulist[u];
+        if (u < cap) {
+            const PairRec& rec = stacks[r.stack].recs[r.slot];
+            tiles[u] = make_int4(rec.k1, u, 1 | (r.stack << 4), -1);
+            plist[u] = make_int4(rec.k2, r.slot, u, 0);
+        } else {
+            const int f = atomicAdd(fb_count, 1);
+            fb_list[f] = r;
+        }
     }
 }
 
@@ -740,14 +782,11 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
 // One workgroup per pair.  Candidates are enumerated in the reference's tie
 // order q = (j*2L + l2p)*L + l1; the best (value, q) wins.
 struct Exact64Args {
-    Geom g;
-    const double2* P64;
-    const double2* ph64;   // [S][R]
-    const double* shiftv;  // [S]
-    const int* list;       // slots to compute (null: slot = blockIdx-strided 0..n-1)
-    int n;
-    int K;
-    int32_t* cl1; int32_t* cl2; int32_t* jidx; double* corr; double* shift1d;
+    const StackDev* stacks;   // the whole table (SlotRef.stack indexes it)
+    int stack0, nstacks;      // without a list: stacks [stack0, stack0 + nstacks)
+    const SlotRef* list;      // pairs to compute, or null: every pair of every stack
+    const int* n_dev;         // list length (device)
+    long long n_all;          // total pairs when list is null
 };
 
 __device__ inline void slot_to_pair(long long s, int K, int& k1, int& k2) {
@@ -766,22 +805,29 @@ __device__ inline void slot_to_pair(long long s, int K, int& k1, int& k2) {
 __global__ __launch_bounds__(256) void exact64_kernel(Exact64Args a) {
     __shared__ double s_v[256];
     __shared__ long long s_k[256];
-    const Geom g = a.g;
-    const long long NQ = (long long)g.S * 2 * g.L * g.L;
-    for (int it = blockIdx.x; it < a.n; it += gridDim.x) {
-        const int slot = a.list ? a.list[it] : it;
+    const long long n = a.list ? (long long)*a.n_dev : a.n_all;
+    for (long long it = blockIdx.x; it < n; it += gridDim.x) {
+        int stk, slot;
+        if (a.list) { stk = a.list[it].stack; slot = a.list[it].slot; }
+        else {
+            stk = a.stack0 + (a.nstacks > 1 ? find_stack(a.stacks + a.stack0, a.nstacks, it) : 0);
+            slot = (int)(it - a.stacks[stk].pair_base);
+        }
+        const StackDev& sd = a.stacks[stk];
+        const Geom& g = sd.g;
+        const long long NQ = (long long)g.S * 2 * g.L * g.L;
         int k1, k2;
-        slot_to_pair(slot, a.K, k1, k2);
+        slot_to_pair(slot, g.K, k1, k2);
         double bv = -INFINITY;
         long long bk = 0x7fffffffffffffffLL;
-        for (long long q = threadIdx.x; q < NQ; q += blockDim.x) {
-            const int l1 = (int)(q % g.L);
-            const int l2p = (int)((q / g.L) % (2 * g.L));
-            const int j = (int)(q / ((long long)2 * g.L * g.L));
-            const double2* P1 = a.P64 + ((size_t)k1 * g.L + l1) * g.R;
-            const double2* P2 = a.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
-            const double v = entry64(P1, P2, a.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
-            if (!isnan(v) && better64(v, q, bv, bk)) { bv = v; bk = q; }
+        for (long long qq = threadIdx.x; qq < NQ; qq += blockDim.x) {
+            const int l1 = (int)(qq % g.L);
+            const int l2p = (int)((qq / g.L) % (2 * g.L));
+            const int j = (int)(qq / ((long long)2 * g.L * g.L));
+            const double2* P1 = sd.P64 + ((size_t)k1 * g.L + l1) * g.R;
+            const double2* P2 = sd.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
+            const double v = entry64(P1, P2, sd.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
+            if (!isnan(v) && better64(v, qq, bv, bk)) { bv = v; bk = qq; }
         }
         s_v[threadIdx.x] = bv;
         s_k[threadIdx.x] = bk;
@@ -795,26 +841,7 @@ __global__ __launch_bounds__(256) void exact64_kernel(Exact64Args a) {
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) {
-            const double v = s_v[0];
-            const long long q = s_k[0];
-            if (v > 0.0) {
-                const int l1 = (int)(q % g.L);
-                const int l2p = (int)((q / g.L) % (2 * g.L));
-                const int j = (int)(q / ((long long)2 * g.L * g.L));
-                if (a.cl1) a.cl1[slot] = l1 + 1;
-                if (a.cl2) a.cl2[slot] = l2p + 1;
-                if (a.jidx) a.jidx[slot] = j;
-                if (a.corr) a.corr[slot] = v;
-                if (a.shift1d) a.shift1d[slot] = a.shiftv[j];
-            } else {
-                if (a.cl1) a.cl1[slot] = 0;
-                if (a.cl2) a.cl2[slot] = 0;
-                if (a.jidx) a.jidx[slot] = -1;
-                if (a.corr) a.corr[slot] = 0.0;
-                if (a.shift1d) a.shift1d[slot] = 0.0;
-            }
-        }
+        if (threadIdx.x == 0) write_result(sd, slot, s_v[0], s_k[0]);
         __syncthreads();
     }
 }

@@ -45,6 +45,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--flags", type=int, default=0, help="clm_params.reserved bits (1: no lagged schedule)")
     ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy"],
                     help="pairs: one BF16 call on the workload and nothing else (for counter "
                          "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration)")
@@ -82,7 +83,7 @@ def main():
     W_pair = 8.0 * S * L * L * R  # algorithmic flops per pair
 
     def run(prec, x, variant=0):
-        return clmatrix_records(x, ms, step, precision=prec, variant=variant)
+        return clmatrix_records(x, ms, step, precision=prec, variant=variant, flags=args.flags)
 
     if args.mode == "pairs":
         rec, stats = run(args.precision, pf, args.variant)

@@ -67,7 +67,7 @@ typedef struct clm_params {
     double tie_margin;   /* FP64 re-score of candidates within this of the
                             FP32 best (correlation units), default 1e-3       */
     int32_t variant;     /* 0 = tuned/default kernel variant, else a variant id */
-    int32_t reserved;
+    int32_t reserved;    /* tuning bits; bit 0 disables the lagged wave schedule */
 } clm_params;
 
 typedef struct clm_stats {
@@ -127,6 +127,51 @@ int clm_clmatrix(clm_ctx *ctx, const clm_params *p, const void *pf,
 int clm_clmatrix_device(clm_ctx *ctx, const clm_params *p, const void *d_pf,
                         int32_t *d_cl1, int32_t *d_cl2, int32_t *d_jidx,
                         double *d_corr, double *d_shift1d, clm_stats *stats);
+
+/*
+ * Several independent stacks in one device job (batched cryo_clmatrix calls,
+ * e.g. the per-class or per-dataset calls a pipeline makes in a loop around
+ * cryo_clmatrix, abinitio/C1/cryo_abinitio_C1_worker.m:99).  Stacks may
+ * differ in K, n_r, n_theta, shift grid and precision; stacks whose kernel
+ * variant agrees share one persistent launch.  io[i] holds stack i's device
+ * pointers (pf and outputs, as for clm_clmatrix_device); p[i] its
+ * parameters.  stats (optional) is the sum over stacks.
+ * flags: CLM_GROUP_GRAPH captures the job in a HIP graph on the first call
+ * and replays it on later calls with identical parameters and pointers (the
+ * graph holds no memset nodes: counters are cleared by a kernel).
+ */
+typedef struct clm_stack_io {
+    const void *pf;
+    int32_t *cl1, *cl2, *jidx;
+    double *corr, *shift1d;
+} clm_stack_io;
+
+#define CLM_GROUP_GRAPH 1
+
+int clm_clmatrix_grouped(clm_ctx *ctx, int32_t n, const clm_params *p,
+                         const clm_stack_io *io, int32_t flags, clm_stats *stats);
+
+/*
+ * Autotuner (no counterpart in the reference, whose GPU path has no tunable
+ * kernel).  Times every pair-kernel variant that fits the shape of p -- output
+ * tile width (32 x CT columns per wave), pairs per workgroup and the lagged
+ * wave schedule -- on the first sample_pairs pairs (<= 0: 200000) of pf
+ * (device memory, as for clm_clmatrix_device), and keeps the fastest per pair
+ * for later calls on this context whose variant is 0 and whose precision,
+ * n_r, n_theta and shift grid match.  entries (capacity max_entries, may be
+ * NULL when 0) receive each variant's timing; *n_entries their count.
+ * Leaves the outputs of earlier calls untouched.
+ */
+typedef struct clm_tune_entry {
+    int32_t variant;  /* CT*100 + waves per workgroup                */
+    int32_t lag;      /* 1 = lagged wave schedule                     */
+    double ms;        /* pass-1 kernel time on the sample             */
+    int64_t pairs;    /* pairs in the sample                          */
+} clm_tune_entry;
+
+int clm_autotune(clm_ctx *ctx, const clm_params *p, const void *d_pf,
+                 int32_t sample_pairs, clm_tune_entry *entries,
+                 int32_t max_entries, int32_t *n_entries);
 
 /*
  * Host-side FP64 assembly of the reference outputs from per-pair records
