@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic variant)")
+    ap.add_argument("--tune-pairs", type=int, default=200000)
     ap.add_argument("--flags", type=int, default=0, help="clm_params.reserved bits (1: no lagged schedule)")
     ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy"],
                     help="pairs: one BF16 call on the workload and nothing else (for counter "
@@ -67,7 +69,7 @@ def main():
         return
 
     from aspire_amd import sim
-    from aspire_amd.clmatrix import clmatrix_records
+    from aspire_amd.clmatrix import autotune, clmatrix_records
 
     n, n_r, T, ms, step, snr = 129, 65, 360, 10.0, 0.5, 1 / 16
     K = args.K
@@ -84,6 +86,14 @@ def main():
 
     def run(prec, x, variant=0):
         return clmatrix_records(x, ms, step, precision=prec, variant=variant, flags=args.flags)
+
+    tune = None
+    if not args.no_tune and args.variant == 0:
+        t = time.perf_counter()
+        rows, best = autotune(pf, ms, step, args.precision, sample_pairs=args.tune_pairs, flags=args.flags)
+        tune = {"sample_pairs": args.tune_pairs, "seconds": time.perf_counter() - t, "chosen": best,
+                "variants": rows}
+        log(f"autotune ({tune['seconds']:.1f}s): {best}")
 
     if args.mode == "pairs":
         rec, stats = run(args.precision, pf, args.variant)
@@ -109,11 +119,20 @@ def main():
     value = P / sec
     k_ms = float(np.mean(kms))
     achieved = W_pair * P / (k_ms * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pair_kernel_k5000_traffic.json")
+    if os.path.exists(tpath) and K == 5000:
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     # plain FP32 GPU path on the first fp32-K images of the same stack
     Kf = min(args.fp32_K, K)
     pf_f = pf[:, :, :Kf]
     Pf = Kf * (Kf - 1) // 2
+    tune32 = None
+    if not args.no_tune:
+        rows32, best32 = autotune(pf_f, ms, step, "fp32", sample_pairs=min(args.tune_pairs, Pf), flags=args.flags)
+        tune32 = {"chosen": best32, "variants": rows32}
     _ = run("fp32", pf_f)
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -183,11 +202,13 @@ def main():
                    "K": K, "pairs": P, "n_r": n_r, "n_theta": T, "n_shifts": S,
                    "precision": args.precision, "variant": all_stats[-1]["variant"]},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": BF16_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK, "traffic": traffic,
                      "kernel": "pair_kernel (pass 1)", "kernel_ms": k_ms,
                      "flops_per_pair": W_pair, "executed_flops_per_pair": W_pair / 2},
         "cpu_baseline": cpu,
-        "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "ms_pairs": st32["ms_pairs"]},
+        "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "ms_pairs": st32["ms_pairs"],
+                     "variant": st32["variant"], "autotune": tune32},
+        "autotune": tune,
         "speedup_vs_fp32": value / fp32_value,
         "clstack_match_pct": 100.0 * float(same.mean()),
         "parity_sample": {"pairs": int(ns), "disagree": int((~same).sum()), "near_tie": int(near),
