@@ -25,7 +25,9 @@ PRECISIONS = {
 EXPORTS = (
     "clm_version", "clm_default_params", "clm_num_pairs", "clm_create",
     "clm_destroy", "clm_last_error", "clm_clmatrix", "clm_clmatrix_device",
-    "clm_clmatrix_grouped", "clm_autotune", "clm_assemble", "clm_shift_equations_csc",
+    "clm_clmatrix_grouped", "clm_autotune", "clm_clmatrix_windowed",
+    "clm_shiftnet_nparam", "clm_shiftnet_features", "clm_shiftnet_forward",
+    "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
 )
 
 CLM_GROUP_GRAPH = 1
@@ -65,6 +67,23 @@ class ClmTuneEntry(ctypes.Structure):
         ("ms", ctypes.c_double),
         ("pairs", ctypes.c_int64),
     ]
+
+
+class ClmWindowStats(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_int64),
+        ("accepted", ctypes.c_int64),
+        ("below_gate", ctypes.c_int64),
+        ("off_grid", ctypes.c_int64),
+        ("window_shifts", ctypes.c_int32),
+        ("full_shifts", ctypes.c_int32),
+        ("ms_window", ctypes.c_double),
+        ("ms_fallback", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class ClmStats(ctypes.Structure):
@@ -129,6 +148,20 @@ def load():
     L.clm_autotune.argtypes = [vp, ctypes.POINTER(ClmParams), vp, ctypes.c_int32,
                                ctypes.POINTER(ClmTuneEntry), ctypes.c_int32,
                                ctypes.POINTER(ctypes.c_int32)]
+    L.clm_clmatrix_windowed.restype = ctypes.c_int
+    L.clm_clmatrix_windowed.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, ctypes.c_double, ctypes.c_double,
+                                        vp, vp, vp, vp, vp, vp, ctypes.POINTER(ClmWindowStats)]
+    L.clm_shiftnet_nparam.restype = ctypes.c_int
+    L.clm_shiftnet_nparam.argtypes = []
+    L.clm_shiftnet_features.restype = ctypes.c_int
+    L.clm_shiftnet_features.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp]
+    L.clm_shiftnet_forward.restype = ctypes.c_int
+    L.clm_shiftnet_forward.argtypes = [vp, vp, vp, ctypes.c_int32, vp]
+    L.clm_shiftnet_grad.restype = ctypes.c_int
+    L.clm_shiftnet_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int32, vp, vp]
+    L.clm_shiftnet_adam.restype = ctypes.c_int
+    L.clm_shiftnet_adam.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float]
     L.clm_assemble.restype = ctypes.c_int
     L.clm_assemble.argtypes = [ctypes.c_int32, vp, vp, vp, vp, vp, vp]
     L.clm_shift_equations_csc.restype = ctypes.c_int

@@ -1,7 +1,10 @@
 """Build libclm.so (HIP, gfx950) in-tree with hipcc.
 
-``python -m aspire_amd.build`` compiles ``aspire_amd/csrc/clm_api.hip`` (which
-includes the kernels) into ``aspire_amd/libclm.so``.
+``python -m aspire_amd.build`` compiles, in parallel, ``aspire_amd/csrc/
+clm_api.hip`` (the engine: context, orchestration and its kernels),
+``shiftnet.hip`` (the shift network) and ``pair_variants.hip`` once per
+(operand type, shift-grid symmetry), and links them into
+``aspire_amd/libclm.so``.
 """
 from __future__ import annotations
 
@@ -13,7 +16,12 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libclm.so")
-SRC = os.path.join(PKG, "csrc", "clm_api.hip")
+CSRC = os.path.join(PKG, "csrc")
+# (source, extra flags): the engine, the shift network, and the pair-kernel
+# instantiations, one object per (operand type, shift-grid symmetry)
+UNITS = [(os.path.join(CSRC, "clm_api.hip"), []), (os.path.join(CSRC, "shiftnet.hip"), [])] + [
+    (os.path.join(CSRC, "pair_variants.hip"), [f"-DCLM_VARIANT_OP={op}", f"-DCLM_VARIANT_SYM={sym}"])
+    for op in ("bf16", "f16", "f32") for sym in (1, 0)]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -21,7 +29,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # fixed per-element arithmetic in the FP64 stages (bit-parity with the
     # oracle); hot FP32 code uses explicit fmaf where a fused op is wanted
     "-ffp-contract=off",
@@ -40,15 +47,34 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> str:
     """Compile libclm.so; returns its path.  Always recompiles."""
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
-           "-I", os.path.join(PKG, "csrc"), "-o", tmp, SRC]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+    import tempfile
+
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc")]
+    with tempfile.TemporaryDirectory(prefix="clm_build_") as tmpdir:
+        objs, procs = [], []
+        for n, (src, extra) in enumerate(UNITS):
+            obj = os.path.join(tmpdir, f"{n}_{os.path.basename(src)}.o")
+            cmd = [hipcc(), *HIPCC_FLAGS, *extra, *inc, "-c", "-o", obj, src]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+            objs.append(obj)
+        failed = None
+        for cmd, pr in procs:
+            out, err = pr.communicate()
+            if pr.returncode != 0:
+                sys.stderr.write(out + err)
+                failed = failed or pr.returncode
+        if failed:
+            raise RuntimeError(f"hipcc failed with exit code {failed}")
+        tmp = LIB + ".tmp"
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,6 +1,7 @@
 // clm_api.hip -- C ABI of libclm (declared in include/clm.h): context,
 // geometry, orchestration of the device passes, host-side FP64 assembly.
 #include "clm.h"
+#include "clm_ctx.h"
 #include "clm_kernels.hip"
 
 #include <algorithm>
@@ -12,90 +13,9 @@
 #include <vector>
 
 using namespace clm;
+using namespace clmh;
 
 namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-};
-
-}  // namespace
-
-struct StackBufs {
-    DevBuf P64, P32, Bop, nrm, astk, recs, ph32, ph64, shiftv, H;
-};
-
-// Pass-1 tiles of one group, kept while the group's shapes repeat.
-struct TileCache {
-    std::vector<long long> key;  // (table index, K, PPW) per member
-    DevBuf tiles;
-    int ntiles = 0;
-};
-
-struct clm_ctx {
-    int device = 0;
-    int n_cu = 0;
-    int lds_max = 65536;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::vector<StackBufs> sb;          // per stack of the current call
-    std::vector<TileCache> tcache;      // most recent first
-    DevBuf table, ctr, ulist, erecs, plist, tiles2, fblist;
-    DevBuf pf, out_cl1, out_cl2, out_j, out_corr, out_shift;  // host-buffer form
-    hipEvent_t ev[6] = {};
-    // autotuned variants: shape key -> (ct, W, lag)
-    std::vector<std::pair<unsigned long long, int4>> tuned;
-    DevBuf tune_tiles;
-    // captured device job of the grouped form, replayed while the signature repeats
-    std::vector<unsigned long long> gsig;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-};
-
-namespace {
-
-int fail(clm_ctx* c, int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    if (c) c->err = buf;
-    return code;
-}
-
-#define HIPCHK(ctx, expr)                                                              \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail((ctx), CLM_E_HIP, "%s failed: %s (%s:%d)", #expr,              \
-                        hipGetErrorString(e_), __FILE__, __LINE__);                    \
-    } while (0)
-
-int ensure(clm_ctx* c, DevBuf& b, size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    if (b.n >= bytes) return CLM_OK;
-    if (b.p) {
-        hipError_t e = hipFree(b.p);
-        b.p = nullptr;
-        b.n = 0;
-        if (e != hipSuccess) return fail(c, CLM_E_HIP, "hipFree: %s", hipGetErrorString(e));
-    }
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        return fail(c, CLM_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
-    }
-    b.n = bytes;
-    return CLM_OK;
-}
-
-void release(DevBuf& b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.n = 0;
-}
 
 // Operand element kinds (prep_kernel's op_kind)
 enum { OPK_F32 = 0, OPK_BF16 = 1, OPK_F16 = 2 };
@@ -140,82 +60,16 @@ Variant choose_variant(int opk, int NCT, int Rp, int want_ct, int want_W) {
     return best;
 }
 
-using PairFn = void (*)(PairArgs);
-
-template <typename OpT, int CT, int NK, bool SYM>
-PairFn pk(bool emit) {
-    return emit ? pair_kernel<OpT, CT, NK, SYM, true> : pair_kernel<OpT, CT, NK, SYM, false>;
-}
-template <typename OpT, int CT, bool SYM>
-PairFn pick_nk(int NK, bool emit) {
-    switch (NK) {
-        case 1: return pk<OpT, CT, 1, SYM>(emit);
-        case 2: return pk<OpT, CT, 2, SYM>(emit);
-        case 3: return pk<OpT, CT, 3, SYM>(emit);
-        case 4: return pk<OpT, CT, 4, SYM>(emit);
-        case 6: return pk<OpT, CT, 6, SYM>(emit);
-        case 8: return pk<OpT, CT, 8, SYM>(emit);
-    }
-    return nullptr;
-}
-template <int CT, bool SYM>
-PairFn pick_f32(int NK, bool emit) {
-    switch (NK) {
-        case 8: return pk<float, CT, 8, SYM>(emit);
-        case 16: return pk<float, CT, 16, SYM>(emit);
-        case 24: return pk<float, CT, 24, SYM>(emit);
-        case 32: return pk<float, CT, 32, SYM>(emit);
-        case 48: return pk<float, CT, 48, SYM>(emit);
-        case 64: return pk<float, CT, 64, SYM>(emit);
-    }
-    return nullptr;
-}
-template <typename OpT, bool SYM>
-PairFn pick_ct(int ct, int NK, bool emit) {
-    switch (ct) {
-        case 1: return pick_nk<OpT, 1, SYM>(NK, emit);
-        case 2: return pick_nk<OpT, 2, SYM>(NK, emit);
-        case 3: return pick_nk<OpT, 3, SYM>(NK, emit);
-    }
-    return nullptr;
-}
-
 PairFn pick_pair_kernel(int opk, int ct, int Rp, bool sym, bool emit) {
-    if (opk == OPK_F32) {
-        const int NK = Rp / 2;
-        if (ct == 1) return sym ? pick_f32<1, true>(NK, emit) : pick_f32<1, false>(NK, emit);
-        if (ct == 2 && NK <= 32) return sym ? pick_f32<2, true>(NK, emit) : pick_f32<2, false>(NK, emit);
-        return nullptr;
-    }
-    const int NK = Rp / 16;
-    if (opk == OPK_BF16) return sym ? pick_ct<__bf16, true>(ct, NK, emit) : pick_ct<__bf16, false>(ct, NK, emit);
-    return sym ? pick_ct<_Float16, true>(ct, NK, emit) : pick_ct<_Float16, false>(ct, NK, emit);
-}
-
-using AstkFn = void (*)(Geom, const float2*, const float2*, int4*);
-
-template <typename OpT, bool SYM>
-AstkFn pick_astk_nk(int NK) {
-    switch (NK) {
-        case 1: return astack_kernel<OpT, 1, SYM>;
-        case 2: return astack_kernel<OpT, 2, SYM>;
-        case 3: return astack_kernel<OpT, 3, SYM>;
-        case 4: return astack_kernel<OpT, 4, SYM>;
-        case 6: return astack_kernel<OpT, 6, SYM>;
-        case 8: return astack_kernel<OpT, 8, SYM>;
-        case 16: return astack_kernel<OpT, 16, SYM>;
-        case 24: return astack_kernel<OpT, 24, SYM>;
-        case 32: return astack_kernel<OpT, 32, SYM>;
-        case 48: return astack_kernel<OpT, 48, SYM>;
-        case 64: return astack_kernel<OpT, 64, SYM>;
-    }
-    return nullptr;
+    if (opk == OPK_F32) return sym ? pick_pair_f32_1(ct, Rp / 2, emit) : pick_pair_f32_0(ct, Rp / 2, emit);
+    if (opk == OPK_BF16) return sym ? pick_pair_bf16_1(ct, Rp / 16, emit) : pick_pair_bf16_0(ct, Rp / 16, emit);
+    return sym ? pick_pair_f16_1(ct, Rp / 16, emit) : pick_pair_f16_0(ct, Rp / 16, emit);
 }
 
 AstkFn pick_astk(int opk, int Rp, bool sym) {
-    if (opk == OPK_F32) return sym ? pick_astk_nk<float, true>(Rp / 2) : pick_astk_nk<float, false>(Rp / 2);
-    if (opk == OPK_BF16) return sym ? pick_astk_nk<__bf16, true>(Rp / 16) : pick_astk_nk<__bf16, false>(Rp / 16);
-    return sym ? pick_astk_nk<_Float16, true>(Rp / 16) : pick_astk_nk<_Float16, false>(Rp / 16);
+    if (opk == OPK_F32) return sym ? pick_astk_f32_1(Rp / 2) : pick_astk_f32_0(Rp / 2);
+    if (opk == OPK_BF16) return sym ? pick_astk_bf16_1(Rp / 16) : pick_astk_bf16_0(Rp / 16);
+    return sym ? pick_astk_f16_1(Rp / 16) : pick_astk_f16_0(Rp / 16);
 }
 
 inline long long pair_slot(long long k1, long long k2, long long K) {
@@ -329,11 +183,19 @@ struct StackJob {
     int lag = 0;
     long long P = 0;
     int group = -1;
+    const double* lsh = nullptr;     // per-line pre-shift [K][L] (windowed search) or null
+    // selected pairs only (device-built): list of (stack, slot), count, pass-1 tiles
+    const SlotRef* sel_list = nullptr;
+    const int* sel_count = nullptr;
+    const int4* sel_tiles = nullptr;
+    const int4* sel_plist = nullptr;
+    const int* sel_ntiles = nullptr;
 };
 
 // Stacks that share one pair-kernel instantiation (or the FP64 kernel).
 struct Group {
     int opk = 0, exact = 0, ct = 0, Rp = 0, sym = 0, W = 0, CPP = 0, lag = 0;
+    int sel = 0;                // one stack of selected pairs (device-built tiles and list)
     std::vector<int> members;   // job indices
     int first = 0;              // first table index
     long long P = 0;            // pairs of all members
@@ -349,6 +211,7 @@ struct Plan {
     std::vector<Group> groups;
     std::vector<int> order;      // table index -> job index
     std::vector<StackDev> table;
+    int sb_base = 0;             // first per-stack buffer set of the context used
     long long P = 0;
     int n_unc_cap = 0;           // entries of the shared pass-2 buffers
 };
@@ -401,11 +264,14 @@ int plan_stack(clm_ctx* c, const clm_params* p, StackJob& j) {
 
 // Allocate and upload everything the device job reads; build the table.
 // Nothing here is captured: the job itself (enqueue_job) only launches kernels.
-int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_io* ios, bool build_tiles = true) {
+int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_io* ios, bool build_tiles = true,
+            const StackJob* extra = nullptr, int sb_base = 0) {
     int rc;
     pl.jobs.resize(n);
+    pl.sb_base = sb_base;
     for (int i = 0; i < n; i++) {
         StackJob& j = pl.jobs[i];
+        if (extra) j = extra[i];
         j.p = ps[i];
         j.io = ios[i];
         if ((rc = plan_stack(c, &ps[i], j))) {
@@ -421,9 +287,9 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         const Geom& g = j.s.g;
         const int lag = j.lag;
         int gi = -1;
-        for (size_t q = 0; q < pl.groups.size(); q++) {
+        for (size_t q = 0; q < pl.groups.size() && !j.sel_list; q++) {
             const Group& G = pl.groups[q];
-            if (G.exact == j.exact && (j.exact || (G.opk == j.opk && G.ct == j.v.ct && G.Rp == g.Rp &&
+            if (!G.sel && G.exact == j.exact && (j.exact || (G.opk == j.opk && G.ct == j.v.ct && G.Rp == g.Rp &&
                                                    G.sym == g.sym && G.W == j.v.W && G.CPP == j.v.CPP &&
                                                    G.lag == lag))) {
                 gi = (int)q;
@@ -434,6 +300,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
             Group G;
             G.opk = j.opk; G.exact = j.exact; G.ct = j.v.ct; G.Rp = g.Rp; G.sym = g.sym;
             G.W = j.v.W; G.CPP = j.v.CPP; G.lag = lag;
+            G.sel = j.sel_list != nullptr;
             pl.groups.push_back(G);
             gi = (int)pl.groups.size() - 1;
         }
@@ -456,11 +323,11 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         pl.P += G.P;
     }
     const int nt = (int)pl.order.size();
-    if (c->sb.size() < (size_t)nt) c->sb.resize(nt);
+    if (c->sb.size() < (size_t)(sb_base + nt)) c->sb.resize(sb_base + nt);
     pl.table.assign(nt, StackDev{});
     for (int ti = 0; ti < nt; ti++) {
         StackJob& j = pl.jobs[pl.order[ti]];
-        StackBufs& B = c->sb[ti];
+        StackBufs& B = c->sb[pl.sb_base + ti];
         const Geom& g = j.s.g;
         const long long K = g.K;
         const size_t osz = j.opk == OPK_F32 ? 4 : 2;
@@ -528,7 +395,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         G.occ = 0;
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&G.occ, (const void*)G.fn, 64 * G.W, G.lds));
         if (G.occ < 1) G.occ = 1;
-        if (!build_tiles) continue;
+        if (!build_tiles || G.sel) continue;
         const int PPW = G.W / G.CPP;
         std::vector<long long> key;
         for (int ti = G.first; ti < G.first + (int)G.members.size(); ti++) {
@@ -561,7 +428,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
     }
     // tile caches can move while inserting: resolve pointers afterwards
     for (Group& G : pl.groups) {
-        if (G.exact || !build_tiles) continue;
+        if (G.exact || !build_tiles || G.sel) continue;
         const int PPW = G.W / G.CPP;
         std::vector<long long> key;
         for (int ti = G.first; ti < G.first + (int)G.members.size(); ti++) {
@@ -583,7 +450,7 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
     unsigned long long* ctr = (unsigned long long*)c->ctr.p;
     for (size_t ti = 0; ti < pl.order.size(); ti++) {
         const StackJob& j = pl.jobs[pl.order[ti]];
-        const StackBufs& B = c->sb[ti];
+        const StackBufs& B = c->sb[pl.sb_base + ti];
         const Geom& g = j.s.g;
         if ((rc = enqueue_fill(c, B.nrm.p, g.K, 0))) return rc;
         const long long lines = (long long)g.K * g.Lp;
@@ -591,11 +458,11 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
         if (j.p.pf_dtype == CLM_C128)
             hipLaunchKernelGGL(prep_kernel<double>, dim3(nb), dim3(256), 0, c->stream, (const double*)j.io.pf, g,
                                (const double*)B.H.p, (double2*)B.P64.p, (float2*)B.P32.p, B.Bop.p, j.opk,
-                               (unsigned*)B.nrm.p, ctr + 0);
+                               (unsigned*)B.nrm.p, ctr + 0, j.lsh);
         else
             hipLaunchKernelGGL(prep_kernel<float>, dim3(nb), dim3(256), 0, c->stream, (const float*)j.io.pf, g,
                                (const double*)B.H.p, (double2*)B.P64.p, (float2*)B.P32.p, B.Bop.p, j.opk,
-                               (unsigned*)B.nrm.p, ctr + 0);
+                               (unsigned*)B.nrm.p, ctr + 0, j.lsh);
         HIPCHK(c, hipGetLastError());
         if (!j.exact) {
             const size_t osz = j.opk == OPK_F32 ? 4 : 2;
@@ -635,6 +502,10 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
             ea.list = nullptr;
             ea.n_dev = nullptr;
             ea.n_all = G.P;
+            if (G.sel) {
+                ea.list = pl.jobs[G.members[0]].sel_list;
+                ea.n_dev = pl.jobs[G.members[0]].sel_count;
+            }
             const unsigned nb = (unsigned)std::min<long long>(G.P, (long long)c->n_cu * 8);
             hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
             HIPCHK(c, hipGetLastError());
@@ -643,15 +514,26 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
         unsigned long long* cg = ctr + 16 + 4 * gi;
         PairArgs pa{};
         pa.stacks = dtab;
-        pa.tiles = (const int4*)G.tc->tiles.p;
-        pa.ntiles = G.tc->ntiles;
-        pa.ntiles_dev = nullptr;
         pa.counter = (int*)(cg + 0);
-        pa.plist = nullptr;
         pa.erecs = nullptr;
         pa.CPP = G.CPP;
         pa.lag = G.lag;
-        const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, G.tc->ntiles);
+        long long max_tiles;
+        if (G.sel) {
+            const StackJob& j = pl.jobs[G.members[0]];
+            pa.tiles = j.sel_tiles;
+            pa.ntiles = 0;
+            pa.ntiles_dev = j.sel_ntiles;
+            pa.plist = j.sel_plist;
+            max_tiles = G.P;  // at least one pair per tile
+        } else {
+            pa.tiles = (const int4*)G.tc->tiles.p;
+            pa.ntiles = G.tc->ntiles;
+            pa.ntiles_dev = nullptr;
+            pa.plist = nullptr;
+            max_tiles = G.tc->ntiles;
+        }
+        const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, max_tiles);
         hipLaunchKernelGGL(G.fn, dim3(grid), dim3(64 * G.W), G.lds, c->stream, pa);
         HIPCHK(c, hipGetLastError());
     }
@@ -668,6 +550,13 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
         ra.nstacks = (int)G.members.size();
         ra.mode = 0;
         ra.n_items = G.P;
+        if (G.sel) {
+            const StackJob& j = pl.jobs[G.members[0]];
+            ra.mode = 2;
+            ra.n_items_dev = j.sel_count;
+            ra.item_cap = (int)G.P;
+            ra.ulist = j.sel_list;
+        }
         ra.unc_count = (int*)(cg + 1);
         ra.unc_list = ul;
         ra.fb_count = (int*)(ctr + 4);
@@ -774,7 +663,7 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
     }
     add(pl.table.data(), pl.table.size() * sizeof(StackDev));
     for (size_t ti = 0; ti < pl.order.size(); ti++) {
-        const StackBufs& B = c->sb[ti];
+        const StackBufs& B = c->sb[pl.sb_base + ti];
         const void* sp[] = {B.P64.p, B.P32.p, B.Bop.p, B.nrm.p, B.astk.p, B.recs.p, B.ph32.p, B.ph64.p, B.shiftv.p, B.H.p};
         add(sp, sizeof sp);
     }
@@ -826,6 +715,115 @@ int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bo
     rc = read_stats(c, pl, false, st);
     if (st) st->ms_total = event_ms(c->ev[0], c->ev[3]);
     return rc;
+}
+
+// Learned form: exhaustive search cut to a window around each pair's shift
+// as predicted from per-image 2-D shift proposals, with the exhaustive search
+// as the fallback for pairs whose window peak is below `gate` or lands off
+// the reference grid.
+int run_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float* d_prop, double window,
+                 double gate, const clm_stack_io& out, uint8_t* d_flag, clm_window_stats* ws) {
+    int rc;
+    if (ws) std::memset(ws, 0, sizeof *ws);
+    Setup sf;
+    if ((rc = setup(c, p, sf))) return rc;
+    const double step = p->shift_step;
+    const double q = p->max_shift / step, wq = window / step;
+    if (std::fabs(q - std::rint(q)) > 1e-9)
+        return fail(c, CLM_E_UNSUP, "windowed search needs max_shift (%g) to be a multiple of shift_step (%g)",
+                    p->max_shift, step);
+    if (!(window >= 0) || std::fabs(wq - std::rint(wq)) > 1e-9)
+        return fail(c, CLM_E_ARG, "window (%g) must be a non-negative multiple of shift_step (%g)", window, step);
+    if (p->precision == CLM_FP64_REF) return fail(c, CLM_E_UNSUP, "windowed search runs the MFMA precisions");
+    const long long K = p->K, P = K * (K - 1) / 2;
+    const int L = sf.g.L;
+    if (ws) {
+        ws->pairs = P;
+        ws->full_shifts = sf.g.S;
+        ws->window_shifts = 2 * (int)std::rint(wq) + 1;
+    }
+    if (P == 0) return CLM_OK;
+    if (!d_pf || !d_prop) return fail(c, CLM_E_ARG, "null pf or proposals");
+    // full-grid variant first: its pairs per workgroup shape the fallback tiles
+    StackJob probe;
+    if ((rc = plan_stack(c, p, probe))) return rc;
+    const int ppw = probe.v.W / probe.v.CPP;
+    if ((rc = ensure(c, c->w_cl1, P * 4)) || (rc = ensure(c, c->w_cl2, P * 4)) || (rc = ensure(c, c->w_j, P * 4)) ||
+        (rc = ensure(c, c->w_corr, P * 8)) || (rc = ensure(c, c->w_shift, P * 8)) ||
+        (rc = ensure(c, c->w_lsh, (size_t)K * L * 8)) || (rc = ensure(c, c->w_flag, (size_t)P)) ||
+        (rc = ensure(c, c->w_rows, (size_t)(4 * K + 4) * 4)) ||
+        (rc = ensure(c, c->w_list, (size_t)P * sizeof(SlotRef))) ||
+        (rc = ensure(c, c->w_plist, (size_t)P * sizeof(int4))) ||
+        (rc = ensure(c, c->w_tiles, (size_t)P * sizeof(int4))) ||
+        (rc = ensure(c, c->w_counts, 8 * sizeof(unsigned long long))))
+        return rc;
+    if ((rc = upload(c, c->w_shiftv, sf.shifts))) return rc;
+    int* rows = (int*)c->w_rows.p;
+    int *cnt = rows, *tcnt = rows + K, *off = rows + 2 * K, *toff = rows + 3 * K, *tot = rows + 4 * K;
+    unsigned long long* counts = (unsigned long long*)c->w_counts.p;
+
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = enqueue_fill(c, counts, 16, 0)) || (rc = enqueue_fill(c, tot, 2, 0))) return rc;
+    hipLaunchKernelGGL(lineshift_kernel, dim3((unsigned)((K * L + 255) / 256)), dim3(256), 0, c->stream, d_prop,
+                       (int)K, L, step, (double*)c->w_lsh.p);
+    HIPCHK(c, hipGetLastError());
+    // windowed search: the same engine on rays moved per line, 2*window/step+1 shifts
+    clm_params pw = *p;
+    pw.max_shift = window;
+    StackJob xw;
+    xw.lsh = (const double*)c->w_lsh.p;
+    const clm_stack_io iow{d_pf, (int32_t*)c->w_cl1.p, (int32_t*)c->w_cl2.p, (int32_t*)c->w_j.p,
+                           (double*)c->w_corr.p, (double*)c->w_shift.p};
+    {
+        Plan plw;
+        if ((rc = prepare(c, plw, 1, &pw, &iow, true, &xw, 0))) return rc;
+        if ((rc = enqueue_job(c, plw, false))) return rc;
+        hipLaunchKernelGGL(window_merge_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream, (int)K,
+                           L, (const double*)c->w_lsh.p, (const int32_t*)c->w_cl1.p, (const int32_t*)c->w_cl2.p,
+                           (const double*)c->w_corr.p, (const double*)c->w_shift.p, gate, p->max_shift, step,
+                           sf.g.S, (const double*)c->w_shiftv.p, out.cl1, out.cl2, out.jidx, out.corr, out.shift1d,
+                           (uint8_t*)c->w_flag.p, counts);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(row_count_kernel, dim3((unsigned)(K - 1)), dim3(256), 0, c->stream, (int)K,
+                           (const uint8_t*)c->w_flag.p, ppw, cnt, tcnt);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(row_scan_kernel, dim3(1), dim3(1024), 0, c->stream, (int)(K - 1), (const int*)cnt,
+                           (const int*)tcnt, off, toff, tot);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(row_emit_kernel, dim3((unsigned)(K - 1)), dim3(256), 0, c->stream, (int)K, 0,
+                           (const uint8_t*)c->w_flag.p, ppw, (const int*)off, (const int*)toff,
+                           (SlotRef*)c->w_list.p, (int4*)c->w_plist.p, (int4*)c->w_tiles.p);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // the window plan's shared buffers are reused below
+    }
+    // exhaustive search of the flagged pairs, results into the same outputs
+    StackJob xf;
+    xf.sel_list = (const SlotRef*)c->w_list.p;
+    xf.sel_count = tot;
+    xf.sel_tiles = (const int4*)c->w_tiles.p;
+    xf.sel_plist = (const int4*)c->w_plist.p;
+    xf.sel_ntiles = tot + 1;
+    {
+        Plan plf;
+        if ((rc = prepare(c, plf, 1, p, &out, true, &xf, 1))) return rc;
+        if (plf.jobs[0].v.W / plf.jobs[0].v.CPP != ppw) return fail(c, CLM_E_HIP, "fallback tile shape changed");
+        if ((rc = enqueue_job(c, plf, false))) return rc;
+    }
+    if (d_flag) HIPCHK(c, hipMemcpyAsync(d_flag, c->w_flag.p, (size_t)P, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (ws) {
+        unsigned long long h[4] = {0};
+        HIPCHK(c, hipMemcpy(h, counts, sizeof h, hipMemcpyDeviceToHost));
+        ws->accepted = (int64_t)h[0];
+        ws->below_gate = (int64_t)h[1];
+        ws->off_grid = (int64_t)h[2];
+        ws->ms_window = event_ms(c->ev[0], c->ev[1]);
+        ws->ms_fallback = event_ms(c->ev[1], c->ev[2]);
+        ws->ms_total = event_ms(c->ev[0], c->ev[2]);
+    }
+    return CLM_OK;
 }
 
 // Candidate variants of a shape: every (CT, W) the kernels support, W a
@@ -976,6 +974,11 @@ void clm_destroy(clm_ctx* c) {
     }
     for (TileCache& tc : c->tcache) release(tc.tiles);
     release(c->tune_tiles);
+    DevBuf* snb[] = {&c->sn_h1, &c->sn_h2, &c->sn_z, &c->sn_y, &c->sn_dy, &c->sn_dz, &c->sn_dz2, &c->sn_dz1};
+    for (DevBuf* b : snb) release(*b);
+    DevBuf* wb[] = {&c->w_cl1, &c->w_cl2, &c->w_j, &c->w_corr, &c->w_shift, &c->w_lsh, &c->w_flag, &c->w_rows,
+                    &c->w_list, &c->w_plist, &c->w_tiles, &c->w_counts, &c->w_shiftv};
+    for (DevBuf* b : wb) release(*b);
     DevBuf* bufs[] = {&c->table, &c->ctr, &c->ulist, &c->erecs, &c->plist, &c->tiles2, &c->fblist, &c->pf,
                       &c->out_cl1, &c->out_cl2, &c->out_j, &c->out_corr, &c->out_shift};
     for (DevBuf* b : bufs) release(*b);
@@ -995,6 +998,16 @@ int clm_clmatrix_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32
     HIPCHK(c, hipSetDevice(c->device));
     const clm_stack_io io{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
     return run_job(c, 1, p, &io, false, st);
+}
+
+int clm_clmatrix_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float* d_prop, double window,
+                          double gate, int32_t* d_cl1, int32_t* d_cl2, int32_t* d_jidx, double* d_corr,
+                          double* d_shift1d, uint8_t* d_flag, clm_window_stats* ws) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    HIPCHK(c, hipSetDevice(c->device));
+    const clm_stack_io out{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
+    return run_windowed(c, p, d_pf, d_prop, window, gate, out, d_flag, ws);
 }
 
 int clm_autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t sample_pairs, clm_tune_entry* entries,

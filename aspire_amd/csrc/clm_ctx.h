@@ -1,0 +1,104 @@
+// clm_ctx.h -- host-side context of libclm shared by its translation units:
+// the context (one HIP stream, grow-only device buffers, caches) and the
+// error/allocation helpers every C-ABI entry point uses.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "clm.h"
+
+namespace clmh {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+struct StackBufs {
+    DevBuf P64, P32, Bop, nrm, astk, recs, ph32, ph64, shiftv, H;
+};
+
+// Pass-1 tiles of one group, kept while the group's shapes repeat.
+struct TileCache {
+    std::vector<long long> key;  // (table index, K, PPW) per member
+    DevBuf tiles;
+    int ntiles = 0;
+};
+
+}  // namespace clmh
+
+struct clm_ctx {
+    int device = 0;
+    int n_cu = 0;
+    int lds_max = 65536;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<clmh::StackBufs> sb;          // per stack of the current call
+    std::vector<clmh::TileCache> tcache;      // most recent first
+    clmh::DevBuf table, ctr, ulist, erecs, plist, tiles2, fblist;
+    clmh::DevBuf pf, out_cl1, out_cl2, out_j, out_corr, out_shift;  // host-buffer form
+    hipEvent_t ev[6] = {};
+    // windowed search: window records, per-line shifts, flags, selected-pair lists
+    clmh::DevBuf w_cl1, w_cl2, w_j, w_corr, w_shift, w_lsh, w_flag, w_rows, w_list, w_plist, w_tiles, w_counts,
+        w_shiftv;
+    // shift network: activations and gradients of the last batch
+    clmh::DevBuf sn_h1, sn_h2, sn_z, sn_y, sn_dy, sn_dz, sn_dz2, sn_dz1;
+    // autotuned variants: shape key -> (ct, W, lag)
+    std::vector<std::pair<unsigned long long, int4>> tuned;
+    clmh::DevBuf tune_tiles;
+    // captured device job of the grouped form, replayed while the signature repeats
+    std::vector<unsigned long long> gsig;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace clmh {
+
+inline int fail(clm_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::clmh::fail((ctx), CLM_E_HIP, "%s failed: %s (%s:%d)", #expr,      \
+                                hipGetErrorString(e_), __FILE__, __LINE__);            \
+    } while (0)
+
+inline int ensure(clm_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.n >= bytes) return CLM_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        if (e != hipSuccess) return fail(c, CLM_E_HIP, "hipFree: %s", hipGetErrorString(e));
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, CLM_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    b.n = bytes;
+    return CLM_OK;
+}
+
+inline void release(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+
+}  // namespace clmh

@@ -1,4 +1,5 @@
-// clm_kernels.hip -- gfx950 kernels of the common-lines engine.
+// clm_kernels.hip -- gfx950 kernels of the common-lines engine (the pair
+// kernel and A-stack templates are in clm_pair.h).
 //
 // The hot path of ASPIRE's cryo_clmatrix_cpu (sinograms/cryo_clmatrix_cpu.m:
 // 296-399) for every pair (k1,k2), every shift s_j and every pair of lines:
@@ -21,40 +22,9 @@
 //                    re-score of near ties, reference tie order.
 //   exact64_kernel   FP64 exhaustive search of one pair (fallback and the
 //                    CLM_FP64_REF path); the reference's own arithmetic.
-#include "clm_device.h"
+#include "clm_pair.h"
 
 namespace clm {
-
-// ---------------------------------------------------------------- operands
-template <typename OpT> struct Op;
-
-template <> struct Op<__bf16> {
-    using frag = bf16x8;
-    static constexpr int KSTEP = 16;  // k per MFMA
-    __device__ static inline f32x16 mma(frag a, frag b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct Op<_Float16> {
-    using frag = f16x8;
-    static constexpr int KSTEP = 16;
-    __device__ static inline f32x16 mma(frag a, frag b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct Op<float> {
-    using frag = float;
-    static constexpr int KSTEP = 2;
-    __device__ static inline f32x16 mma(frag a, frag b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
-    }
-};
-
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // -------------------------------------------------------------- preprocess
 // One thread per line (k,l).  Sequential FP64 arithmetic in the oracle's
@@ -67,7 +37,8 @@ __global__ void prep_kernel(const InT* __restrict__ pf, Geom g,
                             void* __restrict__ Bop,        // [K][2][Lp][Rp]
                             int op_kind,                   // 0 f32, 1 bf16, 2 f16
                             unsigned* __restrict__ nrm_bits,  // [K] float bits
-                            unsigned long long* __restrict__ n_small) {
+                            unsigned long long* __restrict__ n_small,
+                            const double* __restrict__ lsh) { // [K][L] per-line shift (px) or null
     // lines run over the padded range l < Lp so padding rows of the operand
     // planes are written (zeros) by the same launch
     const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,10 +72,20 @@ __global__ void prep_kernel(const InT* __restrict__ pf, Geom g,
     double2* o64 = P64 ? P64 + ((size_t)k * L + l) * R : nullptr;
     float2* o32 = P32 + ((size_t)k * L + l) * Rp;
     double hn = 0.0;
+    const double a = lsh ? lsh[(size_t)k * L + l] : 0.0;
     for (int r = 0; r < R; r++) {
         const int rr = n_r - 1 - r;
         double re = ((double)zc[2 * rr] * H[r]) / nr;
         double im = ((double)zc[2 * rr + 1] * H[r]) / nr;
+        if (lsh) {
+            // line moved by a px: times exp(-2*pi*i*rk*a/(2R+1)), rk = r - R,
+            // the phase the shift search applies for shift a
+            const double th = ((-2.0 * M_PI * (double)(r - R)) * a) / (double)(2 * R + 1);
+            const double c = cos(th), sn = sin(th);
+            const double xr = re * c - im * sn, xi = re * sn + im * c;
+            re = xr;
+            im = xi;
+        }
         if (o64) o64[r] = make_double2(re, im);
         float fr = (float)re, fi = (float)im;
         o32[r] = make_float2(fr, fi);
@@ -124,398 +105,16 @@ __global__ void prep_kernel(const InT* __restrict__ pf, Geom g,
     if (isfinite(hf)) atomicMax(nrm_bits + k, __float_as_uint(hf));
 }
 
-// ------------------------------------------------------------- A-stacks
-// Image k's shift-expanded rays ("A-stack"): rows (e, l1) of P_k .* ph_e,
-// rounded once to the operand type, 32 rows per A-tile, each row stored as
-// CPR 16-byte chunks (re half, then im half).  Chunk c of row r is stored at
-// position (c + r) mod CPR, so the MFMA fragment reads (32 rows at the same
-// logical chunk) hit distinct LDS banks without row padding.  Rows past
-// the last shift are zero.  An A-tile is one contiguous block that the pair
-// kernel copies to LDS with direct global-to-LDS loads.
-template <typename OpT, int NK> struct ATile {
-    static constexpr int Rp = NK * Op<OpT>::KSTEP;
-    static constexpr int CPR = 2 * Rp * (int)sizeof(OpT) / 16;  // chunks per row
-    static constexpr int EPC = 16 / (int)sizeof(OpT);            // elements per chunk
-    static constexpr int NCH = 32 * CPR;                         // chunks per tile
-    static constexpr int BYTES = NCH * 16;
-    __device__ static inline int phys(int row, int c) { return (c + row) % CPR; }
-};
-
-template <typename OpT, int NK, bool SYM>
-__global__ void astack_kernel(Geom g, const float2* __restrict__ P32, const float2* __restrict__ ph32,
-                              int4* __restrict__ astk) {
-    using AT = ATile<OpT, NK>;
-    constexpr int Rp = AT::Rp, CPR = AT::CPR, EPC = AT::EPC;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long)g.K * g.NAT * AT::NCH) return;
-    const int pch = (int)(q % CPR);
-    long r = q / CPR;
-    const int row = (int)(r % 32);
-    r /= 32;
-    const int at = (int)(r % g.NAT);
-    const int k = (int)(r / g.NAT);
-    const int c = (pch - row % CPR + CPR) % CPR;   // logical chunk stored here
-    const bool imag = c >= CPR / 2;
-    const int k0 = (c % (CPR / 2)) * EPC;
-    const int gr = at * 32 + row;
-    OpT v[EPC];
-    if (gr < g.NROWS) {
-        const int e = gr / g.L, l1 = gr - e * g.L;
-        const float sg = (!SYM && e >= g.S) ? -1.0f : 1.0f;
-        const float2* p = P32 + ((size_t)k * g.L + l1) * Rp + k0;
-        const float2* ph = ph32 + (size_t)e * Rp + k0;
-#pragma unroll
-        for (int u = 0; u < EPC; u++) {
-            const float2 pv = p[u], qv = ph[u];
-            const float re = fmaf(pv.x, qv.x, -pv.y * qv.y);
-            const float im = fmaf(pv.x, qv.y, pv.y * qv.x);
-            v[u] = (OpT)(imag ? sg * im : re);
-        }
-    } else {
-#pragma unroll
-        for (int u = 0; u < EPC; u++) v[u] = (OpT)0.0f;
-    }
-    int4 out;
-    __builtin_memcpy(&out, v, 16);
-    astk[q] = out;
-}
-
-// -------------------------------------------------------------- pair kernel
-struct PairArgs {
-    const StackDev* stacks;
-    const int4* tiles;     // {k1, k2_first | list offset, count | stack << 4, slot | -1}
-    int ntiles;            // tile count, or
-    const int* ntiles_dev; // (when set) tile count in device memory
-    int* counter;          // next tile (persistent work queue)
-    const int4* plist;     // explicit pairs {k2, slot, u, 0}
-    EmitRec* erecs;        // indexed by u (second pass)
-    int CPP;               // waves per pair (column chunks); same for all stacks
-    int lag;               // 1: one wave per SIMD runs one A-tile behind
-};
-
-constexpr int WCAP = 64;   // per-wave list of candidate cells (one per lane)
-
-__device__ inline unsigned long long lanemask_lt() {
-    const int lane = threadIdx.x & 63;
-    return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
-// Maximum over the 64 lanes with DPP row operations and 4 lane reads.
-__device__ inline float wave_max_dpp(float v) {
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0xb1, 0xf, 0xf, false)));  // quad_perm [1,0,3,2]
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x4e, 0xf, 0xf, false)));  // quad_perm [2,3,0,1]
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x124, 0xf, 0xf, false))); // row_ror:4
-    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x128, 0xf, 0xf, false))); // row_ror:8
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
-}
-
-// Dynamic LDS bytes of pair_kernel (the A-tile ring is static).
-__host__ __device__ inline size_t pair_lds_bytes(int W) { return 384 + (size_t)W * WCAP * 8; }
-
-// Persistent all-pairs kernel.  A workgroup takes a tile (one image k1 and up
-// to W/CPP partner images k2).  k1's A-tiles (32 rows (e, l1) of its
-// shift-expanded rays each, precomputed by astack_kernel) stream through two
-// LDS buffers, prefetched one tile ahead through registers; each wave
-// multiplies every A-tile against its k2's columns, which stay in
-// registers.  The epilogue reduces each lane's 32x32 output tile to
-// one "cell" maximum of t = X + |Y|.
-//   EMIT = false (pass 1): each wave keeps every cell that comes within the
-//     certified margin of the pair's running maximum; at the end the cells
-//     within the margin of the pair's final maximum go to the record.
-//   EMIT = true (pass 2): every candidate at or above the pair's threshold T
-//     from pass 1 is listed.
-template <typename OpT, int CT, int NK, bool SYM, bool EMIT>
-__global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
-    using O = Op<OpT>;
-    using frag = typename O::frag;
-    using AT = ATile<OpT, NK>;
-    constexpr int Rp = AT::Rp, CPR = AT::CPR, NCH = AT::NCH;
-    constexpr int NPT = (NCH + 255) / 256;    // chunks per thread (W >= 4)
-    __shared__ __align__(16) int4 sa0[NCH];
-    __shared__ __align__(16) int4 sa1[NCH];
-    extern __shared__ __align__(16) char smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int W = blockDim.x >> 6;
-    const int h = lane >> 5, lc = lane & 31;
-    int* s_i = (int*)smem;                    // [0] tile, [1..] per-pair counters
-    float* s_pm = (float*)(s_i + 32);         // per-wave running maxima
-    int* s_unc = (int*)(s_pm + 32);           // per-pair flags
-    float2* s_list = (float2*)(s_unc + 32);   // [W][WCAP] (value, id bits)
-    float2* wlist = s_list + wave * WCAP;
-    const int pi = wave / a.CPP, chunk = wave % a.CPP;
-    const int wv = __builtin_amdgcn_readfirstlane(wave);
-
-    // Which waves lag: the second wave (by index) on each SIMD, from the SIMD
-    // id in HW_REG_HW_ID (bits 5:4), shared through LDS.
-    bool lag_wave = false;
-    if (a.lag) {
-        const unsigned hwid = __builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 4);
-        if (lane == 0) s_unc[wave] = (int)((hwid >> 4) & 3);
-        __syncthreads();
-        const int my = s_unc[wv];
-        for (int w = 0; w < wv; w++)
-            if (s_unc[w] == my) lag_wave = true;
-        __syncthreads();
-    }
-    const bool lag = a.lag && lag_wave;
-    const int ntiles = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
-
-    for (;;) {
-        if (tid == 0) s_i[0] = atomicAdd(a.counter, 1);
-        if (tid >= 1 && tid < 32) s_i[tid] = 0;
-        if (tid < 32) { s_unc[tid] = 0; s_pm[tid] = -INFINITY; }
-        __syncthreads();
-        const int t = s_i[0];
-        if (t >= ntiles) break;
-        const int4 tile = a.tiles[t];
-        const int k1 = tile.x;
-        const int cnt_t = tile.z & 15;
-        const StackDev& sd = a.stacks[tile.z >> 4];
-        const Geom g = sd.g;
-        const bool active = pi < cnt_t;
-        int k2 = 0, slot = 0, uidx = 0;
-        if (active) {
-            if (tile.w >= 0) { k2 = tile.y + pi; slot = tile.w + pi; }
-            else { int4 pe = a.plist[tile.y + pi]; k2 = pe.x; slot = pe.y; uidx = pe.z; }
-        }
-        // A-tiles of image k1 stream through two LDS buffers: each thread
-        // loads its 16-byte chunks of tile at+1 into registers one step
-        // ahead and writes them to LDS at the start of the next step.
-        const int4* asrc = (const int4*)((const char*)sd.astk + (size_t)k1 * g.NAT * AT::BYTES);
-        int4 pre[NPT];
-        auto load_pre = [&](int at) {
-            const int4* src = asrc + (size_t)at * NCH;
-#pragma unroll
-            for (int j = 0; j < NPT; j++) {
-                const int q = tid + j * (int)blockDim.x;
-                if (q < NCH) pre[j] = src[q];
-            }
-        };
-        auto store_pre = [&](int4* buf) {
-#pragma unroll
-            for (int j = 0; j < NPT; j++) {
-                const int q = tid + j * (int)blockDim.x;
-                if (q < NCH) buf[q] = pre[j];
-            }
-        };
-        load_pre(0);
-        // ---- B operand (image k2, this wave's CT column tiles) in registers
-        frag Bre[CT][NK], Bim[CT][NK];
-        {
-            const OpT* B = (const OpT*)sd.Bop;
-            const size_t plane = (size_t)g.Lp * Rp;
-#pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-                const int col = (chunk * CT + ct) * 32 + lc;
-                const OpT* br = B + ((size_t)k2 * 2 + 0) * plane + (size_t)col * Rp;
-                const OpT* bi = br + plane;
-#pragma unroll
-                for (int ks = 0; ks < NK; ks++) {
-                    if constexpr (O::KSTEP == 16) {
-                        Bre[ct][ks] = *(const frag*)(br + 16 * ks + 8 * h);
-                        Bim[ct][ks] = *(const frag*)(bi + 16 * ks + 8 * h);
-                    } else {
-                        Bre[ct][ks] = br[h * (Rp / 2) + ks];
-                        Bim[ct][ks] = bi[h * (Rp / 2) + ks];
-                    }
-                }
-            }
-        }
-        const float n1 = sd.nrm[k1], n2 = sd.nrm[k2];
-        const float margin = 2.0f * (sd.coef * n1 * n2 + sd.sub * (n1 + n2)) + sd.tau_t;
-        float thr = -INFINITY;
-        if (EMIT && active) thr = sd.recs[slot].T;
-        float b = -INFINITY;   // lane's running cell maximum
-        float pm = -INFINITY;  // pair's running maximum (all waves, previous A-tile)
-        int wc = 0;            // entries in this wave's cell list
-        bool lost = false;     // a cell within the margin could not be kept
-
-        float m[CT];
-        f32x16 X[CT], Y[CT];
-        // best of both maps over a cell: t = X + |Y|; pass 2 also lists
-        // every candidate of tile tat at or above the threshold
-        auto epi = [&](int ct, int tat) {
-            float mv = -INFINITY;
-#pragma unroll
-            for (int v = 0; v < 16; v++)
-                mv = fmaxf(mv, SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v]);
-            m[ct] = mv;
-            if constexpr (EMIT) {
-                if (mv >= thr) {
-                    const int col = (chunk * CT + ct) * 32 + lc;
-#pragma unroll
-                    for (int v = 0; v < 16; v++) {
-                        const float tv = SYM ? X[ct][v] + fabsf(Y[ct][v]) : X[ct][v] + Y[ct][v];
-                        const int row = tat * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-                        if (tv >= thr && row < g.NROWS && col < g.L) {
-                            const int pos = atomicAdd(&s_i[1 + pi], 1);
-                            if (pos < MAXE) a.erecs[uidx].cand[pos] = (uint32_t)(row * g.Lp + col);
-                        }
-                    }
-                }
-            }
-        };
-        // pass 1: keep the cells of tile tat that come within the margin
-        // of the pair's running maximum; publish this wave's maximum
-        auto keep_cells = [&](int tat) {
-            if constexpr (!EMIT) {
-                float mm = m[0];
-#pragma unroll
-                for (int ct = 1; ct < CT; ct++) mm = fmaxf(mm, m[ct]);
-                b = fmaxf(b, mm);
-                // wave-uniform threshold from the running maxima seen so
-                // far (all <= the pair's final maximum)
-                const float wm = wave_max_dpp(b);
-                if (lane == 0) s_pm[wave] = wm;
-                const float th = fmaxf(wm, pm) - margin;
-                if (__ballot(mm >= th && mm > -INFINITY)) {
-                    // rare once the pair's maximum has been seen
-#pragma unroll
-                    for (int ct = 0; ct < CT; ct++) {
-                        const bool e = (m[ct] >= th) && (m[ct] > -INFINITY);
-                        const unsigned long long mask = __ballot(e);
-                        if (!mask) continue;
-                        const int cnt = __popcll(mask);
-                        if (wc + cnt > WCAP) {
-                            // compact: drop entries below the current threshold
-                            const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
-                            const bool keep = lane < wc && ent.x >= th;
-                            const unsigned long long km = __ballot(keep);
-                            __builtin_amdgcn_wave_barrier();
-                            if (keep) wlist[__popcll(km & lanemask_lt())] = ent;
-                            wc = __popcll(km);
-                            __builtin_amdgcn_wave_barrier();
-                        }
-                        if (e) {
-                            const int pos = wc + __popcll(mask & lanemask_lt());
-                            const uint32_t id = ((uint32_t)(tat * g.NCT + chunk * CT + ct) << 6) | (uint32_t)lane;
-                            if (pos < WCAP) wlist[pos] = make_float2(m[ct], __uint_as_float(id));
-                        }
-                        if (wc + cnt > WCAP) lost = true;
-                        wc = min(wc + cnt, WCAP);
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                }
-            }
-        };
-        // one A-tile: write tile at+1 (loaded during the previous step) to
-        // the other buffer and start loading tile at+2; reduce (lagging
-        // waves: the previous tile), multiply, reduce (leading waves: this
-        // tile)
-        auto step = [&](int at, const int4* cur, int4* nxt) {
-            if (at + 1 < g.NAT) {
-                store_pre(nxt);
-                if (at + 2 < g.NAT) load_pre(at + 2);
-            }
-            if (lag && active && at > 0) {
-#pragma unroll
-                for (int ct = 0; ct < CT; ct++) epi(ct, at - 1);
-                keep_cells(at - 1);
-            }
-            if (active) {
-                const char* arow = (const char*)cur + lc * CPR * 16;
-#pragma unroll
-                for (int ks = 0; ks < NK; ks++) {
-                    frag a0, b0;
-                    if constexpr (O::KSTEP == 16) {
-                        a0 = *(const frag*)(arow + AT::phys(lc, 2 * ks + h) * 16);
-                        b0 = *(const frag*)(arow + AT::phys(lc, CPR / 2 + 2 * ks + h) * 16);
-                    } else {
-                        const int kk = h * (Rp / 2) + ks;
-                        a0 = *(const float*)(arow + AT::phys(lc, kk / 4) * 16 + (kk % 4) * 4);
-                        b0 = *(const float*)(arow + AT::phys(lc, CPR / 2 + kk / 4) * 16 + (kk % 4) * 4);
-                    }
-#pragma unroll
-                    for (int ct = 0; ct < CT; ct++) {
-                        if (ks == 0) {
-                            f32x16 z = {0.f};
-                            X[ct] = O::mma(a0, Bre[ct][0], z);
-                            Y[ct] = O::mma(b0, Bim[ct][0], z);
-                        } else {
-                            X[ct] = O::mma(a0, Bre[ct][ks], X[ct]);
-                            Y[ct] = O::mma(b0, Bim[ct][ks], Y[ct]);
-                        }
-                    }
-                }
-            }
-            if (!lag && active) {
-#pragma unroll
-                for (int ct = 0; ct < CT; ct++) epi(ct, at);
-                keep_cells(at);
-            }
-            __syncthreads();
-            if (!EMIT && active) {
-                for (int c = 0; c < a.CPP; c++) pm = fmaxf(pm, s_pm[pi * a.CPP + c]);
-            }
-        };
-
-        store_pre(sa0);
-        if (g.NAT > 1) load_pre(1);
-        __syncthreads();
-        for (int at = 0; at < g.NAT; at += 2) {
-            step(at, sa0, sa1);
-            if (at + 1 < g.NAT) step(at + 1, sa1, sa0);
-        }
-        if (lag && active) {
-#pragma unroll
-            for (int ct = 0; ct < CT; ct++) epi(ct, g.NAT - 1);
-            keep_cells(g.NAT - 1);
-        }
-        __syncthreads();
-
-        // ---- per-pair finalisation
-        if constexpr (!EMIT) {
-            // s_pm holds every wave's final maximum (written before the last barrier)
-            if (active) {
-                float M = -INFINITY;
-                for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
-                const float T = M - margin;
-                const float2 ent = lane < wc ? wlist[lane] : make_float2(-INFINITY, 0.f);
-                if (lane < wc && ent.x >= T) {
-                    const int pos = atomicAdd(&s_i[1 + pi], 1);
-                    if (pos < MAXC) sd.recs[slot].cell[pos] = __float_as_uint(ent.y);
-                }
-                if (lost) atomicOr(&s_unc[pi], 1);
-            }
-            __syncthreads();
-            if (active && chunk == 0 && lane == 0) {
-                float M = -INFINITY;
-                for (int c = 0; c < a.CPP; c++) M = fmaxf(M, s_pm[pi * a.CPP + c]);
-                const int n = s_i[1 + pi];
-                PairRec* r = sd.recs + slot;
-                r->M = M;
-                r->T = M - margin;
-                r->k1 = k1;
-                r->k2 = k2;
-                r->ncells = n < MAXC ? n : MAXC;
-                r->flags = (n > MAXC ? F_OVERFLOW : 0) | (s_unc[pi] ? F_UNCERTAIN : 0) |
-                           (M == -INFINITY ? F_EMPTY : 0);
-            }
-        } else {
-            __syncthreads();
-            if (active && chunk == 0 && lane == 0) {
-                const int n = s_i[1 + pi];
-                a.erecs[uidx].n = n < MAXE ? n : MAXE;
-                a.erecs[uidx].overflow = n > MAXE;
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------------------ re-scoring
 struct RescoreArgs {
     const StackDev* stacks;   // the whole table (SlotRef.stack indexes it)
     int stack0, nstacks;      // mode 0: stacks [stack0, stack0 + nstacks)
-    int mode;                 // 0: every pair of those stacks (pass 1), 1: pass-2 list
+    int mode;                 // 0: every pair of those stacks (pass-1 records),
+                              // 1: list with pass-2 candidates, 2: list with pass-1 records
     long long n_items;        // mode 0: total pairs of all stacks
-    const int* n_items_dev;   // mode 1: number of list entries (device)
-    int item_cap;             // mode 1: entries beyond this were not given a pass 2
-    const SlotRef* ulist;     // mode 1: u -> (stack, slot)
+    const int* n_items_dev;   // modes 1, 2: number of list entries (device)
+    int item_cap;             // modes 1, 2: entries beyond this are not processed
+    const SlotRef* ulist;     // modes 1, 2: u -> (stack, slot)
     const EmitRec* erecs;     // mode 1: candidates of entry u
     int* unc_count;           // mode 0: uncertain pairs appended here
     SlotRef* unc_list;
@@ -624,10 +223,11 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
     const int W = blockDim.x >> 6;
     float* sval = s_val[wave];
     const int q = lane & 3, rl = lane >> 2;  // k-quarter, row within the step
-    const long long n_items = a.mode == 0 ? a.n_items : (long long)min(*a.n_items_dev, a.item_cap);
+    const bool listed = a.mode != 0, emit = a.mode == 1;
+    const long long n_items = listed ? (long long)min(*a.n_items_dev, a.item_cap) : a.n_items;
     for (long long item = (long long)blockIdx.x * W + wave; item < n_items; item += (long long)gridDim.x * W) {
         int stk, slot;
-        if (a.mode == 0) {
+        if (!listed) {
             stk = a.stack0 + (a.nstacks > 1 ? find_stack(a.stacks + a.stack0, a.nstacks, item) : 0);
             slot = (int)(item - a.stacks[stk].pair_base);
         } else {
@@ -639,9 +239,10 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         const int kq = g.Rp / 4;  // multiple of 4
         const PairRec rec = sd.recs[slot];
         const int k1 = rec.k1, k2 = rec.k2;
-        const EmitRec* er = a.mode ? a.erecs + item : nullptr;
+        const EmitRec* er = emit ? a.erecs + item : nullptr;
+        const int cmode = emit ? 1 : 0;
         int nrows;
-        if (a.mode == 0) {
+        if (!emit) {
             if (rec.flags & F_EMPTY) {
                 if (lane == 0) write_result(sd, slot, 0.0, 0);
                 continue;
@@ -666,7 +267,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         for (int base = 0; base < nrows; base += 16) {
             const int r = base + rl;
             int row = -1, col = 0;
-            if (r < nrows) cand_rc(g, a.mode, rec, er, r, row, col);
+            if (r < nrows) cand_rc(g, cmode, rec, er, r, row, col);
             float X = 0.f, Y = 0.f;
             int e = 0, l1 = 0;
             if (row >= 0) {
@@ -731,7 +332,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
                 const int iv = base + lane;
                 if (iv < nv && sval[iv] >= lim) {
                     int row, col;
-                    cand_rc(g, a.mode, rec, er, iv >> 1, row, col);
+                    cand_rc(g, cmode, rec, er, iv >> 1, row, col);
                     const int e = row / g.L, l1 = row - e * g.L;
                     long long key;
                     if (g.sym) key = (iv & 1) ? tie_key(g.S - 1 - e, g.L + col, l1, g.L) : tie_key(e, col, l1, g.L);
@@ -849,6 +450,155 @@ __global__ __launch_bounds__(256) void exact64_kernel(Exact64Args a) {
 __global__ void fill_i32(int* p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------ windowed search
+// Per-line 1-D shift of image k implied by a proposed 2-D shift (x, y):
+// a(l) = sin(l*pi/L)*x + cos(l*pi/L)*y, the coefficients of the shift
+// equations (cryo_clmatrix_cpu.m:722-738), rounded to a multiple of `step`
+// so that window shifts land on the reference grid.
+__global__ void lineshift_kernel(const float* __restrict__ prop, int K, int L, double step,
+                                 double* __restrict__ lsh) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)K * L) return;
+    const int k = (int)(i / L), l = (int)(i % L);
+    const double th = (double)l * M_PI / (double)L;
+    const double a = sin(th) * (double)prop[2 * k] + cos(th) * (double)prop[2 * k + 1];
+    lsh[i] = rint(a / step) * step;
+}
+
+// Window result -> reference grid.  Line l1 of image k1 was moved by a1 and
+// line l2 of k2 by a2, so window shift s at a C1 column is grid shift
+// s + a1 - a2 and at a C2 column s - a1 - a2.  Pairs whose window peak is
+// below `gate`, or whose grid shift is off the grid, are flagged for the
+// exhaustive search; the others are written in the reference's form.
+__global__ void window_merge_kernel(int K, int L, const double* __restrict__ lsh,
+                                    const int32_t* __restrict__ wcl1, const int32_t* __restrict__ wcl2,
+                                    const double* __restrict__ wcorr, const double* __restrict__ wshift,
+                                    double gate, double max_shift, double step, int S_full,
+                                    const double* __restrict__ shiftv_full,
+                                    int32_t* cl1, int32_t* cl2, int32_t* jidx, double* corr, double* shift1d,
+                                    uint8_t* __restrict__ flag, unsigned long long* __restrict__ counts) {
+    const long long P = (long long)K * (K - 1) / 2;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P) return;
+    int k1, k2;
+    slot_to_pair(e, K, k1, k2);
+    const int c1 = wcl1[e], c2 = wcl2[e];
+    const double v = wcorr[e];
+    int ok = 0, j = -1;
+    double sfull = 0.0;
+    if (c1 > 0 && v >= gate) {
+        const double a1 = lsh[(size_t)k1 * L + (c1 - 1)];
+        const int l2 = (c2 - 1) % L;
+        const double a2 = lsh[(size_t)k2 * L + l2];
+        sfull = c2 <= L ? wshift[e] + a1 - a2 : wshift[e] - a1 - a2;
+        j = (int)rint((sfull + max_shift) / step);
+        ok = j >= 0 && j < S_full;
+    }
+    if (ok) {
+        if (cl1) cl1[e] = c1;
+        if (cl2) cl2[e] = c2;
+        if (jidx) jidx[e] = j;
+        if (corr) corr[e] = v;
+        if (shift1d) shift1d[e] = shiftv_full[j];
+        flag[e] = 0;
+        atomicAdd(counts + 0, 1ull);
+    } else {
+        flag[e] = 1;
+        atomicAdd(counts + (c1 > 0 && v >= gate ? 2 : 1), 1ull);  // 1: below gate, 2: off the grid
+    }
+}
+
+// Flagged pairs -> explicit pass-1 tiles, in pair order.  Three steps: per
+// row counts, one scan, per row ordered compaction.
+__global__ void row_count_kernel(int K, const uint8_t* __restrict__ flag, int ppw, int* __restrict__ cnt,
+                                 int* __restrict__ tcnt) {
+    __shared__ int s_c;
+    const int k1 = blockIdx.x;
+    if (k1 >= K - 1) return;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
+    const long long f = (long long)k1 * K - (long long)k1 * (k1 + 1) / 2;
+    const int n = K - 1 - k1;
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c += flag[f + i] ? 1 : 0;
+    atomicAdd(&s_c, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cnt[k1] = s_c;
+        tcnt[k1] = (s_c + ppw - 1) / ppw;
+    }
+}
+
+// Exclusive scans of cnt and tcnt over n rows (one workgroup of 1024).
+// Totals go to tot[0] (pairs) and tot[1] (tiles).
+__global__ __launch_bounds__(1024) void row_scan_kernel(int n, const int* __restrict__ cnt,
+                                                        const int* __restrict__ tcnt, int* __restrict__ off,
+                                                        int* __restrict__ toff, int* __restrict__ tot) {
+    __shared__ int sa[1024], sb[1024];
+    const int t = threadIdx.x, T = blockDim.x;
+    const int per = (n + T - 1) / T;
+    const int lo = min(n, t * per), hi = min(n, lo + per);
+    int a = 0, b = 0;
+    for (int i = lo; i < hi; i++) { a += cnt[i]; b += tcnt[i]; }
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int o = 1; o < T; o <<= 1) {
+        const int va = t >= o ? sa[t - o] : 0, vb = t >= o ? sb[t - o] : 0;
+        __syncthreads();
+        sa[t] += va;
+        sb[t] += vb;
+        __syncthreads();
+    }
+    int ea = sa[t] - a, eb = sb[t] - b;
+    for (int i = lo; i < hi; i++) {
+        off[i] = ea;
+        toff[i] = eb;
+        ea += cnt[i];
+        eb += tcnt[i];
+    }
+    if (t == T - 1) {
+        tot[0] = sa[t];
+        tot[1] = sb[t];
+    }
+}
+
+__global__ __launch_bounds__(256) void row_emit_kernel(int K, int stack, const uint8_t* __restrict__ flag, int ppw,
+                                                       const int* __restrict__ off, const int* __restrict__ toff,
+                                                       SlotRef* __restrict__ list, int4* __restrict__ plist,
+                                                       int4* __restrict__ tiles) {
+    __shared__ int s_w[4];
+    const int k1 = blockIdx.x;
+    if (k1 >= K - 1) return;
+    const long long f = (long long)k1 * K - (long long)k1 * (k1 + 1) / 2;
+    const int n = K - 1 - k1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int base = off[k1];
+    const int tb = toff[k1], ob = off[k1];
+    for (int i0 = 0; i0 < n; i0 += blockDim.x) {
+        const int i = i0 + threadIdx.x;
+        const bool on = i < n && flag[f + i];
+        const unsigned long long m = __ballot(on);
+        if (lane == 0) s_w[wave] = __popcll(m);
+        __syncthreads();
+        int pre = 0;
+        for (int w = 0; w < wave; w++) pre += s_w[w];
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (on) {
+            const int pos = base + pre + __popcll(m & ((1ull << lane) - 1ull));
+            const int slot = (int)(f + i);
+            list[pos] = SlotRef{stack, slot};
+            plist[pos] = make_int4(k1 + 1 + i, slot, pos, 0);
+        }
+        base += tot;
+        __syncthreads();
+    }
+    // tiles of this row: PPW consecutive listed pairs each
+    const int cnt = base - ob;
+    for (int q = threadIdx.x * ppw; q < cnt; q += blockDim.x * ppw)
+        tiles[tb + q / ppw] = make_int4(k1, ob + q, min(ppw, cnt - q) | (stack << 4), -1);
 }
 
 }  // namespace clm

@@ -1,0 +1,320 @@
+"""Learned form of the shift search: the shift-proposal network and the
+windowed common-lines search built on it.
+
+* :class:`ShiftNet` holds the network's parameters on the GPU (FP32 master
+  weights, their BF16 copy, Adam moments) and drives libclm's hand-written
+  HIP kernels: features, forward, gradient, Adam.
+* :func:`estimate_shifts` recovers per-image 2-D shifts from the exhaustive
+  search's common lines and 1-D shifts by robust least squares on the shift
+  equations (the training targets; cryo_clmatrix_cpu.m:722-741 rows,
+  solved as in sinograms/test_shifts.m:24-27 with iterative reweighting).
+* :func:`train` fits the network; :func:`calibrate_gate` picks the window
+  gate on a stack whose full-grid result is known.
+* :func:`clmatrix_windowed` runs the windowed search with exhaustive
+  fallback (libclm's clm_clmatrix_windowed).
+
+The network and its kernels are described in aspire_amd/csrc/shiftnet.hip.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from .clmatrix import _device_view, _params
+
+NPARAM = 52570
+NR, T = 16, 360
+SHAPES = [("W1", (32, 32, 5)), ("b1", (32,)), ("W2", (8, 32, 5)), ("b2", (8,)),
+          ("F1", (16, 2880)), ("c1", (16,)), ("F2", (2, 16)), ("c2", (2,))]
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "shiftnet.npz")
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-uniform weights (bound sqrt(6 / fan_in)), zero biases; the output
+    layer is scaled by 0.1 so that initial proposals are small."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shp in SHAPES:
+        if len(shp) == 1:
+            parts.append(np.zeros(shp))
+            continue
+        fan_in = int(np.prod(shp[1:]))
+        w = rng.uniform(-1, 1, size=shp) * math.sqrt(6.0 / fan_in)
+        if name == "F2":
+            w *= 0.1
+        parts.append(w)
+    theta = np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+    assert theta.size == NPARAM
+    return theta
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class ShiftNet:
+    """Shift-proposal network on one HIP device."""
+
+    def __init__(self, params=None, device: int = 0, seed: int = 0):
+        import torch
+
+        L = _lib.load()
+        if L.clm_shiftnet_nparam() != NPARAM:
+            raise _lib.ClmError("libclm's shift network has a different parameter layout")
+        self.device = device
+        self.dev = torch.device("cuda", device)
+        theta = init_params(seed) if params is None else np.asarray(params, np.float32)
+        self.master = torch.as_tensor(theta.copy(), device=self.dev)
+        self.w = self.master.to(torch.bfloat16)
+        self.m = torch.zeros_like(self.master)
+        self.v = torch.zeros_like(self.master)
+        self.grad = torch.zeros_like(self.master)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        self.steps = 0
+        self.meta: dict = {}
+        self.ctx = _lib.context(device)
+
+    # --------------------------------------------------------------- kernels
+    def _sync(self):
+        import torch
+
+        torch.cuda.current_stream(self.dev).synchronize()
+
+    def features(self, pf):
+        """x (K x 32 x 360, bf16) from pf (device tensor, column-major
+        n_r x 360 x K as for clmatrix_records)."""
+        import torch
+
+        base, dt, n_r, n_theta, K = _device_view(pf)
+        x = torch.empty((K, 2 * NR, T), dtype=torch.bfloat16, device=self.dev)
+        self._sync()
+        rc = _lib.load().clm_shiftnet_features(self.ctx.handle, _ptr(base), n_r, n_theta, K, dt, _ptr(x))
+        self.ctx.check(rc)
+        return x
+
+    def forward(self, x, chunk: int = 8192):
+        """Proposed shifts in pixels (B x 2, float32 device tensor)."""
+        import torch
+
+        x = x.contiguous()
+        out = torch.empty((x.shape[0], 2), dtype=torch.float32, device=self.dev)
+        self._sync()
+        for b0 in range(0, x.shape[0], chunk):
+            b1 = min(x.shape[0], b0 + chunk)
+            rc = _lib.load().clm_shiftnet_forward(self.ctx.handle, _ptr(self.w), _ptr(x[b0:b1]), b1 - b0,
+                                                  _ptr(out[b0:b1]))
+            self.ctx.check(rc)
+        return out
+
+    def loss_grad(self, x, target_px):
+        """(loss, gradient tensor) of one batch; the gradient is self.grad."""
+        x = x.contiguous()
+        t = target_px.contiguous().float()
+        self._sync()
+        rc = _lib.load().clm_shiftnet_grad(self.ctx.handle, _ptr(self.w), _ptr(x), _ptr(t), x.shape[0],
+                                           _ptr(self.grad), _ptr(self.loss))
+        self.ctx.check(rc)
+        return float(self.loss.item()), self.grad
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.steps += 1
+        rc = _lib.load().clm_shiftnet_adam(self.ctx.handle, _ptr(self.master), _ptr(self.w), _ptr(self.m),
+                                           _ptr(self.v), _ptr(self.grad), self.steps, lr, beta1, beta2, eps)
+        self.ctx.check(rc)
+
+    def propose(self, pf):
+        return self.forward(self.features(pf))
+
+    # ---------------------------------------------------------------- files
+    def params(self) -> np.ndarray:
+        return self.master.detach().cpu().numpy().astype(np.float32)
+
+    def save(self, path: str = WEIGHTS, **meta):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        m = dict(self.meta)
+        m.update(meta)
+        np.savez(path, params=self.params(), meta=np.array(repr(m)))
+
+    @classmethod
+    def load(cls, path: str = WEIGHTS, device: int = 0) -> "ShiftNet":
+        import ast
+
+        with np.load(path, allow_pickle=False) as f:
+            net = cls(params=f["params"], device=device)
+            net.meta = ast.literal_eval(str(f["meta"]))
+        return net
+
+
+# ------------------------------------------------------------------ targets
+def equation_rows(cl1, cl2, L):
+    """Coefficients (sin a, cos a, -sin b, -cos b) or the b >= pi variant of
+    each pair's shift equation (cryo_clmatrix_cpu.m:722-738)."""
+    a = (np.asarray(cl1, np.float64) - 1) * math.pi / L
+    b = (np.asarray(cl2, np.float64) - 1) * math.pi / L
+    low = b < math.pi
+    bb = np.where(low, b, b - math.pi)
+    s = np.where(low, 1.0, -1.0)
+    return np.stack([s * np.sin(a), s * np.cos(a), -np.sin(bb), -np.cos(bb)], axis=1)
+
+
+def estimate_shifts(rec, K: int, L: int, iters: int = 5, weights=None):
+    """2-D shift per image (K x 2) from per-pair records by iteratively
+    reweighted least squares (Cauchy weights) on the shift equations; the
+    minimum-norm solution (LSQR from zero) fixes the 3-dim null space."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import lsqr
+
+    cl1 = np.asarray(rec["cl1"])
+    cl2 = np.asarray(rec["cl2"])
+    rhs = np.asarray(rec["shift"], np.float64)
+    k1, k2 = np.triu_indices(K, 1)
+    ok = cl1 > 0
+    k1, k2, cl1, cl2, rhs = k1[ok], k2[ok], cl1[ok], cl2[ok], rhs[ok]
+    C = equation_rows(cl1, cl2, L)
+    n = k1.size
+    rows = np.repeat(np.arange(n), 4)
+    cols = np.stack([2 * k1, 2 * k1 + 1, 2 * k2, 2 * k2 + 1], axis=1).ravel()
+    A = sp.csr_matrix((C.ravel(), (rows, cols)), shape=(n, 2 * K))
+    w = np.ones(n) if weights is None else np.asarray(weights, np.float64)[ok]
+    x = np.zeros(2 * K)
+    for _ in range(iters):
+        sw = np.sqrt(w)
+        x = lsqr(sp.diags(sw) @ A, sw * rhs, atol=1e-10, btol=1e-10, iter_lim=2000)[0]
+        r = A @ x - rhs
+        scale = 1.4826 * np.median(np.abs(r)) + 1e-6
+        w = 1.0 / (1.0 + (r / (2.385 * scale)) ** 2)
+    return x.reshape(K, 2)
+
+
+# ----------------------------------------------------------------- training
+def make_training_set(n_stacks=20, K=1000, seed0=5001, n=129, n_r=65, input_snrs=(1 / 16, 1 / 4, 1.0),
+                      max_shift_2d=12.0, max_shift=12.0, shift_step=0.5, log=print):
+    """Training pairs (features, shift targets) from n_stacks seeded stacks.
+
+    For every seed the exhaustive search (BF16 engine, full 49-shift grid)
+    runs on the noise-free stack and robust least squares on its shift
+    equations recovers each image's 2-D shift: those are the targets.  The
+    inputs are the same stack (same rotations, phantom and shifts) with white
+    noise at each SNR of ``input_snrs``.  Phantoms are recentred, so each
+    image's centroid is its shift.  Returns (x, targets, truth, per-seed LS
+    rms error in px)."""
+    import torch
+
+    from . import sim
+    from .clmatrix import clmatrix_records
+
+    net = ShiftNet()
+    xs, ys, truth, errs = [], [], [], []
+    for s in range(n_stacks):
+        t0 = time.time()
+        seed = seed0 + s
+        clean = sim.make_stack(K=K, n=n, n_r=n_r, n_theta=T, snr=None, max_shift_2d=max_shift_2d,
+                               shift_kind="uniform", seed=seed, device="cuda", recenter=True)
+        rec, _ = clmatrix_records(clean.pf, max_shift, shift_step, precision="bf16")
+        rec = {k: v.cpu().numpy() for k, v in rec.items()}
+        est = estimate_shifts(rec, K, T // 2)
+        target = torch.as_tensor(est, dtype=torch.float32, device=net.dev)
+        del clean
+        for snr in input_snrs:
+            st = sim.make_stack(K=K, n=n, n_r=n_r, n_theta=T, snr=snr, max_shift_2d=max_shift_2d,
+                                shift_kind="uniform", seed=seed, device="cuda", recenter=True)
+            xs.append(net.features(st.pf))
+            ys.append(target)
+            truth.append(st.shifts)
+            del st
+        err = float(np.sqrt(np.mean(np.sum((est - truth[-1]) ** 2, axis=1))))
+        errs.append(err)
+        log(f"seed {seed}: K={K}, least-squares shift rms error {err:.3f} px ({time.time() - t0:.1f}s)")
+    return torch.cat(xs), torch.cat(ys), np.concatenate(truth), errs
+
+
+def train(net: ShiftNet, x, y, epochs=40, batch=256, lr=1e-3, val_frac=0.1, seed=0, log=print):
+    """Adam on the BF16 network with libclm's kernels.  Returns the
+    per-epoch (train loss, validation rms error in px)."""
+    import torch
+
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    N = x.shape[0]
+    perm = torch.randperm(N, generator=g).to(net.dev)
+    nv = int(N * val_frac)
+    vi, ti = perm[:nv], perm[nv:]
+    hist = []
+    for ep in range(epochs):
+        order = ti[torch.randperm(ti.numel(), generator=g).to(net.dev)]
+        tot, nb = 0.0, 0
+        lr_ep = lr * (0.5 * (1 + math.cos(math.pi * ep / epochs)))
+        for b0 in range(0, order.numel() - batch + 1, batch):
+            idx = order[b0:b0 + batch]
+            loss, _ = net.loss_grad(x.index_select(0, idx), y.index_select(0, idx))
+            net.adam(lr=lr_ep)
+            tot += loss
+            nb += 1
+        pv = net.forward(x.index_select(0, vi)) if nv else None
+        verr = float(torch.sqrt(((pv - y.index_select(0, vi)) ** 2).sum(1).mean())) if nv else float("nan")
+        hist.append((tot / max(nb, 1), verr))
+        log(f"epoch {ep}: loss {tot / max(nb, 1):.5f}, validation rms {verr:.3f} px")
+    return hist
+
+
+def clmatrix_windowed(pf, prop, max_shift=12.0, shift_step=0.5, window=1.0, gate=0.0, precision="bf16",
+                      device=0, variant=0, flags=0):
+    """Windowed search with exhaustive fallback (libclm's
+    clm_clmatrix_windowed).  ``prop``: K x 2 float32 device tensor of
+    proposed shifts in pixels.  Returns (records, window stats); records
+    hold ``exhaustive`` (uint8, 1 = the pair was searched over the full
+    grid)."""
+    import torch
+
+    base, dt, n_r, n_theta, K = _device_view(pf)
+    P = max(K * (K - 1) // 2, 0)
+    dev = base.device
+    out = dict(cl1=torch.zeros(P, dtype=torch.int32, device=dev),
+               cl2=torch.zeros(P, dtype=torch.int32, device=dev),
+               j=torch.zeros(P, dtype=torch.int32, device=dev),
+               corr=torch.zeros(P, dtype=torch.float64, device=dev),
+               shift=torch.zeros(P, dtype=torch.float64, device=dev),
+               exhaustive=torch.zeros(P, dtype=torch.uint8, device=dev))
+    prop = prop.contiguous().float()
+    if prop.shape != (K, 2):
+        raise ValueError("proposals must be K x 2")
+    ctx = _lib.context(device)
+    p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, 0, 1e-3, variant, flags)
+    ws = _lib.ClmWindowStats()
+    torch.cuda.current_stream(dev).synchronize()
+    rc = _lib.load().clm_clmatrix_windowed(ctx.handle, ctypes.byref(p), _ptr(base), _ptr(prop), float(window),
+                                           float(gate), _ptr(out["cl1"]), _ptr(out["cl2"]), _ptr(out["j"]),
+                                           _ptr(out["corr"]), _ptr(out["shift"]), _ptr(out["exhaustive"]),
+                                           ctypes.byref(ws))
+    ctx.check(rc)
+    return out, ws.as_dict()
+
+
+def calibrate_gate(pf, prop, full_rec, max_shift=12.0, shift_step=0.5, window=1.0, target=0.999,
+                   precision="bf16"):
+    """Smallest gate on the window peak such that, among the pairs the
+    window settles, at least ``target`` agree with the full-grid result
+    ``full_rec`` (host arrays).  Returns (gate, table of candidates)."""
+    rec, _ = clmatrix_windowed(pf, prop, max_shift, shift_step, window, gate=-1e30, precision=precision)
+    w = {k: v.cpu().numpy() for k, v in rec.items()}
+    same = (w["cl1"] == full_rec["cl1"]) & (w["cl2"] == full_rec["cl2"]) & (w["j"] == full_rec["j"])
+    # with no gate the window settles every pair whose peak lands on the
+    # grid; at gate g it settles those of them with window corr >= g, and
+    # the rest are searched exhaustively (and then agree by construction)
+    keep = w["exhaustive"] == 0
+    same, corr = same[keep], w["corr"][keep]
+    order = np.argsort(-corr)
+    cum_bad = np.cumsum(~same[order])
+    n_acc = np.arange(1, corr.size + 1)
+    ok = cum_bad <= (1 - target) * n_acc
+    table = []
+    for qn in (0.99, 0.95, 0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.3, 0.2):
+        i = int(qn * (corr.size - 1))
+        table.append((float(corr[order[i]]), float(n_acc[i] / corr.size), float(1 - cum_bad[i] / n_acc[i])))
+    idx = np.flatnonzero(ok)
+    gate = float(corr[order[idx[-1]]]) if idx.size else float(corr.max())
+    return gate, table

@@ -76,15 +76,21 @@ class Blobs:
 
 def gaussian_blobs(rng: np.random.Generator, n_blobs: int = 20, radius: float = 0.5,
                    wmin: float = 0.05, wmax: float = 0.15, amin: float = 0.5,
-                   amax: float = 1.0) -> Blobs:
+                   amax: float = 1.0, recenter: bool = False) -> Blobs:
     """Centers uniform in the ball (direction x radius^(1/3) rule), widths and
-    amplitudes uniform."""
+    amplitudes uniform.  With ``recenter`` every center is moved by the same
+    vector so that the volume's center of mass (blob mass proportional to
+    amplitude x width^3) is the origin: then each projection's centroid is
+    its 2-D shift."""
     v = rng.standard_normal((n_blobs, 3))
     v /= np.linalg.norm(v, axis=1, keepdims=True)
     r = radius * rng.random(n_blobs) ** (1.0 / 3.0)
     centers = v * r[:, None]
     widths = rng.uniform(wmin, wmax, n_blobs)
     amps = rng.uniform(amin, amax, n_blobs)
+    if recenter:
+        mass = amps * widths ** 3
+        centers = centers - (mass[:, None] * centers).sum(0) / mass.sum()
     return Blobs(centers, widths, amps)
 
 
@@ -242,13 +248,13 @@ class Stack:
 def make_stack(K: int, n: int, n_r: int, n_theta: int = 360, snr: float | None = None,
                max_shift_2d: float = 0.0, shift_kind: str = "int", seed: int = 0,
                n_blobs: int = 20, device=None, chunk_images: int = 1024,
-               dtype: str = "complex128") -> Stack:
+               dtype: str = "complex128", recenter: bool = False) -> Stack:
     """A seeded stack: Haar rotations, Gaussian-blob projections, 2-D shifts
     (integer or uniform in [-max_shift_2d, max_shift_2d]), white noise at
     ``snr`` (None = clean) and the polar Fourier transform."""
     rng = np.random.default_rng(seed)
     rots = rand_rots(rng, K)
-    blobs = gaussian_blobs(rng, n_blobs)
+    blobs = gaussian_blobs(rng, n_blobs, recenter=recenter)
     if max_shift_2d > 0:
         if shift_kind == "int":
             shifts = rng.integers(-int(max_shift_2d), int(max_shift_2d) + 1, size=(K, 2)).astype(np.float64)

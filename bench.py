@@ -48,9 +48,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic variant)")
     ap.add_argument("--tune-pairs", type=int, default=200000)
     ap.add_argument("--flags", type=int, default=0, help="clm_params.reserved bits (1: no lagged schedule)")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped"],
                     help="pairs: one BF16 call on the workload and nothing else (for counter "
-                         "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration)")
+                         "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration); "
+                         "grouped: the 64-stack uneven batch in one graph-captured grouped job")
+    ap.add_argument("--stacks", type=int, default=64, help="grouped mode: number of stacks")
+    ap.add_argument("--grouped-sample", type=int, default=40, help="grouped mode: oracle pairs per stack")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -58,6 +61,9 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.mode == "grouped":
+        return grouped_bench(args)
 
     if args.mode == "copy":
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -215,6 +221,88 @@ def main():
                           "corr_rel_max": corr_rel},
         "stats": all_stats[-1],
     }
+    print(json.dumps(out))
+
+
+def grouped_bench(args):
+    """64 independent stacks (seed 4000+j): K ~ U[50,2000], n_r in {33,45,65}
+    (images of 2 n_r - 1 px), n_theta=360, max_shift in {0,4,8} with step 1,
+    SNR in {1,1/8,1/32}, complex64 rays; all stacks in one graph-captured
+    grouped job.  Per-stack clstack agreement with the FP64 oracle on a
+    seeded sample of pairs of every stack."""
+    import torch
+
+    from aspire_amd import sim
+    from aspire_amd.clmatrix import clmatrix_grouped
+    from oracle import clm_oracle
+
+    rng = np.random.default_rng(4000)
+    specs = []
+    for j in range(args.stacks):
+        K = int(rng.integers(50, 2001))
+        n_r = int(rng.choice([33, 45, 65]))
+        ms = int(rng.choice([0, 4, 8]))
+        snr = float(rng.choice([1.0, 1 / 8, 1 / 32]))
+        specs.append((K, n_r, ms, snr))
+    t0 = time.time()
+    pfs = []
+    for j, (K, n_r, ms, snr) in enumerate(specs):
+        st = sim.make_stack(K=K, n=2 * n_r - 1, n_r=n_r, n_theta=360, snr=snr, max_shift_2d=ms / 2,
+                            shift_kind="uniform", seed=4000 + j, device="cuda", dtype="complex64")
+        pfs.append(st.pf)
+    torch.cuda.synchronize()
+    log(f"grouped: {len(pfs)} stacks generated in {time.time() - t0:.1f}s")
+    mss = [s[2] for s in specs]
+    P = sum(s[0] * (s[0] - 1) // 2 for s in specs)
+    recs, st0 = clmatrix_grouped(pfs, mss, 1, "bf16", graph=True)       # capture + first run
+    for i in range(args.warmup):
+        recs, _ = clmatrix_grouped(pfs, mss, 1, "bf16", graph=True, out=recs)
+    times = []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        recs, stats = clmatrix_grouped(pfs, mss, 1, "bf16", graph=True, out=recs)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"grouped step {i}: {times[-1]:.3f}s {stats}")
+    sec = float(np.mean(times))
+    # per-stack parity on a seeded sample
+    srng = np.random.default_rng(42)
+    per, agree_all, n_all, near_all = [], 0, 0, 0
+    cpu_t, cpu_pairs = 0.0, 0
+    for j, (K, n_r, ms, snr) in enumerate(specs):
+        Pj = K * (K - 1) // 2
+        ns = min(args.grouped_sample, Pj)
+        slots = np.sort(srng.choice(Pj, size=ns, replace=False))
+        firsts = np.array([k * K - k * (k + 1) // 2 for k in range(K)], dtype=np.int64)
+        k1 = np.searchsorted(firsts, slots, side="right") - 1
+        k2 = slots - firsts[k1] + k1 + 1
+        imgs = np.unique(np.concatenate([k1, k2]))
+        idx = {int(v): i for i, v in enumerate(imgs)}
+        sub = pfs[j][:, :, torch.as_tensor(imgs, device=pfs[j].device)].cpu().numpy().astype(np.complex128)
+        Psub, _ = clm_oracle.preprocess(sub)
+        kk1 = np.array([idx[int(v)] for v in k1], np.int32)
+        kk2 = np.array([idx[int(v)] for v in k2], np.int32)
+        t = time.perf_counter()
+        ora = clm_oracle.pairs(Psub, ms, 1, kk1, kk2, nthreads=clm_oracle.lib().clmo_max_threads())
+        cpu_t += time.perf_counter() - t
+        cpu_pairs += ns
+        g = {k: v.cpu().numpy()[slots] for k, v in recs[j].items()}
+        same = (g["cl1"] == ora["cl1"]) & (g["cl2"] == ora["cl2"]) & (g["j"] == ora["j"])
+        near = 0
+        for i in np.flatnonzero(~same):
+            v = clm_oracle.entry(Psub, ms, 1, kk1[i], kk2[i], g["j"][i], g["cl1"][i] - 1, g["cl2"][i] - 1)
+            if ora["best"][i] - v <= 1e-5 * abs(ora["best"][i]):
+                near += 1
+        agree_all += int(same.sum())
+        near_all += near
+        n_all += ns
+        per.append({"K": K, "n_r": n_r, "max_shift": ms, "snr": snr, "sample": ns,
+                    "agree": float(same.mean()), "near_tie_disagreements": near})
+    out = {"mode": "grouped", "stacks": len(specs), "pairs": P, "seconds": sec, "pairs_per_s": P / sec,
+           "graph": True, "stats": stats, "clstack_match_pct": 100.0 * agree_all / n_all,
+           "sample_pairs": n_all, "near_tie_disagreements": near_all,
+           "cpu_oracle_pairs_per_s": cpu_pairs / cpu_t, "per_stack": per}
     print(json.dumps(out))
 
 

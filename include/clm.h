@@ -174,6 +174,79 @@ int clm_autotune(clm_ctx *ctx, const clm_params *p, const void *d_pf,
                  int32_t max_entries, int32_t *n_entries);
 
 /*
+ * Learned form: windowed shift search (no counterpart in the reference, whose
+ * search always covers the full grid of cryo_clmatrix_cpu.m:140,360-399).
+ * d_prop holds one proposed 2-D shift per image (K x 2 float, image axes
+ * x, y in pixels, the convention of the shift equations :722-738), e.g. from
+ * the shift network below.  Each line l of image k is moved by its 1-D share
+ * a_k(l) = sin(l*pi/L)*x_k + cos(l*pi/L)*y_k, rounded to a multiple of
+ * shift_step, and the pair search covers only the shifts -window..window
+ * around a1 - a2 (C1 columns) or -a1 - a2 (C2 columns): every window shift is
+ * a shift of the reference grid.  Pairs whose window peak is below `gate`,
+ * or whose best shift lies off the reference grid, are searched again over
+ * the full grid (the exhaustive search is the fallback).  Outputs as for
+ * clm_clmatrix_device; d_flag (optional, P bytes) receives 1 for every pair
+ * that was searched over the full grid, 0 for pairs the window settled.
+ * Requires max_shift and window to be multiples of shift_step; precision
+ * BF16/FP16/FP32.
+ */
+typedef struct clm_window_stats {
+    int64_t pairs;          /* K(K-1)/2                                        */
+    int64_t accepted;       /* pairs settled by the windowed search            */
+    int64_t below_gate;     /* window peak below the gate -> exhaustive        */
+    int64_t off_grid;       /* best shift off the reference grid -> exhaustive */
+    int32_t window_shifts;  /* shifts per window (2*window/step + 1)           */
+    int32_t full_shifts;    /* shifts of the reference grid                    */
+    double ms_window;       /* per-line shifts, windowed search, merge         */
+    double ms_fallback;     /* exhaustive search of the rejected pairs         */
+    double ms_total;
+} clm_window_stats;
+
+int clm_clmatrix_windowed(clm_ctx *ctx, const clm_params *p, const void *d_pf,
+                          const float *d_prop, double window, double gate,
+                          int32_t *d_cl1, int32_t *d_cl2, int32_t *d_jidx,
+                          double *d_corr, double *d_shift1d, uint8_t *d_flag,
+                          clm_window_stats *stats);
+
+/*
+ * Shift network of the learned form (no counterpart in the reference).  It
+ * reads the CLM_SN_NR lowest radial samples of all CLM_SN_T rays of each
+ * image and proposes the image's 2-D shift for clm_clmatrix_windowed.
+ * Two circular 1-D convolutions (32 -> 32 -> 8 channels, kernel 5, ReLU)
+ * and a dense head (2880 -> 16 -> 2); CLM_SN_NPARAM parameters packed as
+ *   W1[32][32][5] b1[32] W2[8][32][5] b2[8] F1[16][2880] c1[16] F2[2][16] c2[2].
+ * Weights and stored activations are BF16 (uint16_t bit patterns), sums are
+ * FP32; gradients and optimizer state FP32.  Every call runs on the
+ * context's stream and returns when its results are in place.
+ *   features: pf (device, as for clm_clmatrix_device; n_theta must be
+ *     CLM_SN_T, n_r >= CLM_SN_NR) -> x, K x 32 x CLM_SN_T BF16: channel 2r+c
+ *     is the real (c=0) or imaginary part of pf(r, t) divided by the root
+ *     mean square of the image's 16 x 360 values.
+ *   forward: x (B images) -> out_px, B x 2 float, proposed shift in pixels.
+ *   grad: loss = 1/(2B) sum |y - target_px/8|^2 with y = out_px/8; writes
+ *     the gradient (CLM_SN_NPARAM floats) and the loss (1 float).
+ *   adam: one Adam step (bias-corrected, step >= 1) on the FP32 master
+ *     weights, refreshing the BF16 copy.
+ */
+#define CLM_SN_NR 16
+#define CLM_SN_T 360
+#define CLM_SN_NPARAM 52570
+
+int clm_shiftnet_nparam(void);
+int clm_shiftnet_features(clm_ctx *ctx, const void *d_pf, int32_t n_r,
+                          int32_t n_theta, int32_t K, int32_t pf_dtype,
+                          uint16_t *d_x);
+int clm_shiftnet_forward(clm_ctx *ctx, const uint16_t *d_w,
+                         const uint16_t *d_x, int32_t B, float *d_out_px);
+int clm_shiftnet_grad(clm_ctx *ctx, const uint16_t *d_w, const uint16_t *d_x,
+                      const float *d_target_px, int32_t B, float *d_grad,
+                      float *d_loss);
+int clm_shiftnet_adam(clm_ctx *ctx, float *d_master, uint16_t *d_w,
+                      float *d_m, float *d_v, const float *d_grad,
+                      int32_t step, float lr, float beta1, float beta2,
+                      float eps);
+
+/*
  * Host-side FP64 assembly of the reference outputs from per-pair records
  * (cryo_clmatrix_cpu.m:392-399,722-741,764-772).  Dense outputs are K x K
  * column-major doubles (MATLAB layout); any may be NULL.
