@@ -101,6 +101,7 @@ class ClmStats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("variant", ctypes.c_int32),
         ("n_eval_shifts", ctypes.c_int32),
+        ("refined_pairs", ctypes.c_int64),
     ]
 
     def as_dict(self):

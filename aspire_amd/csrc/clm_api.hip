@@ -184,6 +184,7 @@ struct StackJob {
     long long P = 0;
     int group = -1;
     const double* lsh = nullptr;     // per-line pre-shift [K][L] (windowed search) or null
+    int force64 = 0;                 // FP64 near-tie re-score even for FP32 operands (refinement)
     // selected pairs only (device-built): list of (stack, slot), count, pass-1 tiles
     const SlotRef* sel_list = nullptr;
     const int* sel_count = nullptr;
@@ -359,9 +360,9 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         const float u = j.opk == OPK_BF16 ? 0x1p-8f : j.opk == OPK_F16 ? 0x1p-11f : 0x1p-24f;
         d.coef = (2.0f * u + u * u + (2.0f * g.Rp + 16.0f) * 0x1p-24f) * 1.01f;
         d.sub = j.opk == OPK_F16 ? 0x1p-24f * std::sqrt(2.0f * g.Rp) : 0.0f;
-        d.tau_t = j.use64 ? (float)(0.5 * j.p.tie_margin) : 0.0f;
+        d.tau_t = (j.use64 || j.force64) ? (float)(0.5 * j.p.tie_margin) : 0.0f;
         d.tau = (float)j.p.tie_margin;
-        d.fp64 = j.use64;
+        d.fp64 = j.use64 || j.force64;
         d.CPP = j.v.CPP;
         d.npairs = j.P;
     }
@@ -480,14 +481,17 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
 // The device job: kernel launches only (no copies, no host waits), so that it
 // can be captured into a HIP graph.  With `timed`, events split it into
 // preprocessing, pair contraction and re-scoring.
-int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
+enum { PHASE_A = 1, PHASE_B = 2, PHASE_ALL = 3 };
+
+int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) {
     int rc;
     unsigned long long* ctr = (unsigned long long*)c->ctr.p;
+    const StackDev* dtab = (const StackDev*)c->table.p;
+    if (phases & PHASE_A) {
     // ctr (u64): 0 small-norm lines, 1 FP64 candidates, 4 fallback count (int),
     // 6 cells re-scored, 7 second-pass candidates; per group at 16+4g:
     // 0 pass-1 tile counter, 1 uncertain count, 2 pass-2 tile counter, 3 pass-2 tiles
     if ((rc = enqueue_fill(c, ctr, 2 * (16 + 4 * (long long)pl.groups.size()), 0))) return rc;
-    const StackDev* dtab = (const StackDev*)c->table.p;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = enqueue_prep(c, pl))) return rc;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -538,7 +542,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
         HIPCHK(c, hipGetLastError());
     }
     if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    // re-score; uncertain pairs get the exact-threshold second pass
+    // re-score; uncertain pairs are listed for the exact-threshold second pass
     for (size_t gi = 0; gi < pl.groups.size(); gi++) {
         const Group& G = pl.groups[gi];
         if (G.exact) continue;
@@ -567,6 +571,23 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
         const unsigned rgrid = (unsigned)std::min<long long>((G.P + 3) / 4, (long long)c->n_cu * 16);
         hipLaunchKernelGGL(rescore_kernel, dim3(rgrid), dim3(256), 0, c->stream, ra);
         HIPCHK(c, hipGetLastError());
+    }
+    }  // phase A
+    if (!(phases & PHASE_B)) return CLM_OK;
+    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+        const Group& G = pl.groups[gi];
+        if (G.exact) continue;
+        unsigned long long* cg = ctr + 16 + 4 * gi;
+        SlotRef* ul = (SlotRef*)c->ulist.p + G.ubase;
+        RescoreArgs ra{};
+        ra.stacks = dtab;
+        ra.stack0 = G.first;
+        ra.nstacks = (int)G.members.size();
+        ra.fb_count = (int*)(ctr + 4);
+        ra.fb_list = (SlotRef*)c->fblist.p;
+        ra.n64 = ctr + 1;
+        ra.ncell = ctr + 6;
+        ra.nemit = ctr + 7;
         const int cap = (int)std::min<long long>(G.P, pl.n_unc_cap);
         hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((G.P + 255) / 256, 1024)),
                            dim3(256), 0, c->stream, dtab, (const int*)(cg + 1), (const SlotRef*)ul, cap,
@@ -610,34 +631,108 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed) {
     return CLM_OK;
 }
 
-int read_stats(clm_ctx* c, const Plan& pl, bool timed, clm_stats* st) {
-    if (!st) return CLM_OK;
+// Add a finished plan's counters to st (which the caller cleared).
+int add_stats(clm_ctx* c, const Plan& pl, clm_stats* st) {
+    if (!st || pl.P == 0) return CLM_OK;
+    std::vector<unsigned long long> h(16 + 4 * pl.groups.size());
+    HIPCHK(c, hipMemcpy(h.data(), c->ctr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    auto as_int = [](unsigned long long x) { return (int64_t)(int)(unsigned)(x & 0xffffffffu); };
+    st->small_norm_lines += (int64_t)h[0];
+    st->fp64_candidates += (int64_t)h[1];
+    st->cells_rescored += (int64_t)h[6];
+    st->emitted += (int64_t)h[7];
+    st->fallback_pairs += as_int(h[4]);
+    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+        if (pl.groups[gi].exact && !pl.groups[gi].sel) st->fallback_pairs += pl.groups[gi].P;
+        else if (!pl.groups[gi].exact) st->uncertain_pairs += as_int(h[16 + 4 * gi + 1]);
+    }
+    return CLM_OK;
+}
+
+void begin_stats(const Plan& pl, clm_stats* st) {
+    if (!st) return;
     std::memset(st, 0, sizeof *st);
     for (const StackJob& j : pl.jobs) st->pairs += j.P;
     if (!pl.jobs.empty()) {
         st->n_eval_shifts = pl.jobs[0].s.g.NE;
         st->variant = pl.jobs[0].v.ct * 100 + pl.jobs[0].v.W;
     }
-    if (pl.P == 0) return CLM_OK;
-    std::vector<unsigned long long> h(16 + 4 * pl.groups.size());
-    HIPCHK(c, hipMemcpy(h.data(), c->ctr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    auto as_int = [](unsigned long long x) { return (int64_t)(int)(unsigned)(x & 0xffffffffu); };
-    st->small_norm_lines = (int64_t)h[0];
-    st->fp64_candidates = (int64_t)h[1];
-    st->cells_rescored = (int64_t)h[6];
-    st->emitted = (int64_t)h[7];
-    st->fallback_pairs = as_int(h[4]);
-    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
-        if (pl.groups[gi].exact) st->fallback_pairs += pl.groups[gi].P;
-        else st->uncertain_pairs += as_int(h[16 + 4 * gi + 1]);
+}
+
+void timed_stats(clm_ctx* c, clm_stats* st) {
+    if (!st) return;
+    st->ms_prep = event_ms(c->ev[0], c->ev[1]);
+    st->ms_pairs = event_ms(c->ev[1], c->ev[2]);
+    st->ms_rescore = event_ms(c->ev[2], c->ev[3]);
+    st->ms_total = event_ms(c->ev[0], c->ev[3]);
+}
+
+// Uncertain pairs of a single-stack plan beyond which the pairs are re-run
+// with FP32 operands instead of the second BF16 pass: a broad correlation
+// peak (clean, high-SNR data) puts many candidates inside the BF16 error
+// bound, which the FP32 bound (16x tighter) cuts down.
+constexpr int REFINE_MIN = 2048;
+
+// Run a plan to completion (no graph).  Single-stack MFMA16 plans whose
+// first pass leaves many pairs uncertain continue with the refinement: the
+// uncertain list is copied aside, and a plan over those pairs only, with
+// FP32 operands (FP64 near-tie re-score kept), settles them into the same
+// outputs.
+int run_plan(clm_ctx* c, const Plan& pl, bool timed, clm_stats* st, int depth = 0) {
+    int rc;
+    const bool can = pl.groups.size() == 1 && !pl.groups[0].exact && pl.jobs.size() == 1 &&
+                     pl.jobs[0].opk != OPK_F32 && depth == 0;
+    if (!can) {
+        if ((rc = enqueue_job(c, pl, timed))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return add_stats(c, pl, st);
     }
-    if (timed) {
-        st->ms_prep = event_ms(c->ev[0], c->ev[1]);
-        st->ms_pairs = event_ms(c->ev[1], c->ev[2]);
-        st->ms_rescore = event_ms(c->ev[2], c->ev[3]);
-        st->ms_total = event_ms(c->ev[0], c->ev[3]);
+    if ((rc = enqueue_job(c, pl, timed, PHASE_A))) return rc;
+    unsigned long long* ctr = (unsigned long long*)c->ctr.p;
+    int n_unc = 0;
+    HIPCHK(c, hipMemcpyAsync(&n_unc, ctr + 16 + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool force = (pl.jobs[0].p.reserved & 2) != 0;  // refine any uncertain pair
+    if (n_unc == 0 || (n_unc < REFINE_MIN && !force)) {
+        if ((rc = enqueue_job(c, pl, timed, PHASE_B))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return add_stats(c, pl, st);
     }
-    return CLM_OK;
+    const StackJob& j0 = pl.jobs[0];
+    const long long P = j0.P;
+    if ((rc = ensure(c, c->cas_list, (size_t)P * sizeof(SlotRef))) || (rc = ensure(c, c->cas_tiles, (size_t)P * 16)) ||
+        (rc = ensure(c, c->cas_plist, (size_t)P * 16)) || (rc = ensure(c, c->cas_cnt, 16)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->cas_list.p, c->ulist.p, (size_t)n_unc * sizeof(SlotRef), hipMemcpyDeviceToDevice,
+                             c->stream));
+    const int cnt[4] = {n_unc, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(c->cas_cnt.p, cnt, sizeof cnt, hipMemcpyHostToDevice, c->stream));
+    // one pair per tile (the FP32 variants of long rays hold one pair per workgroup)
+    hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((n_unc + 255) / 256, 1024)), dim3(256),
+                       0, c->stream, (const StackDev*)c->table.p, (const int*)c->cas_cnt.p,
+                       (const SlotRef*)c->cas_list.p, n_unc, (int4*)c->cas_tiles.p, (int4*)c->cas_plist.p,
+                       (int*)c->cas_cnt.p + 1, (int*)c->cas_cnt.p + 2, (SlotRef*)c->cas_list.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = add_stats(c, pl, st))) return rc;
+    clm_params q = j0.p;
+    q.precision = CLM_FP32;
+    q.variant = 0;
+    StackJob x;
+    x.lsh = j0.lsh;
+    x.force64 = j0.use64;
+    x.sel_list = (const SlotRef*)c->cas_list.p;
+    x.sel_count = (const int*)c->cas_cnt.p;
+    x.sel_tiles = (const int4*)c->cas_tiles.p;
+    x.sel_plist = (const int4*)c->cas_plist.p;
+    x.sel_ntiles = (const int*)c->cas_cnt.p + 1;
+    Plan pr;
+    if ((rc = prepare(c, pr, 1, &q, &j0.io, true, &x, pl.sb_base + 1))) return rc;
+    if ((rc = enqueue_job(c, pr, false))) return rc;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) st->refined_pairs += n_unc;
+    return add_stats(c, pr, st);
 }
 
 void drop_graph(clm_ctx* c) {
@@ -682,14 +777,15 @@ int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bo
     Plan pl;
     int rc = prepare(c, pl, n, ps, ios);
     if (rc) return rc;
+    begin_stats(pl, st);
     if (pl.P == 0) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        return read_stats(c, pl, false, st);
+        return CLM_OK;
     }
     if (!graph) {
-        if ((rc = enqueue_job(c, pl, true))) return rc;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        return read_stats(c, pl, true, st);
+        if ((rc = run_plan(c, pl, true, st))) return rc;
+        timed_stats(c, st);
+        return CLM_OK;
     }
     std::vector<unsigned long long> sig = signature(c, pl);
     if (!c->gexec || sig != c->gsig) {
@@ -712,7 +808,7 @@ int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bo
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    rc = read_stats(c, pl, false, st);
+    rc = add_stats(c, pl, st);
     if (st) st->ms_total = event_ms(c->ev[0], c->ev[3]);
     return rc;
 }
@@ -777,7 +873,7 @@ int run_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float*
     {
         Plan plw;
         if ((rc = prepare(c, plw, 1, &pw, &iow, true, &xw, 0))) return rc;
-        if ((rc = enqueue_job(c, plw, false))) return rc;
+        if ((rc = run_plan(c, plw, false, nullptr))) return rc;
         hipLaunchKernelGGL(window_merge_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream, (int)K,
                            L, (const double*)c->w_lsh.p, (const int32_t*)c->w_cl1.p, (const int32_t*)c->w_cl2.p,
                            (const double*)c->w_corr.p, (const double*)c->w_shift.p, gate, p->max_shift, step,
@@ -806,9 +902,9 @@ int run_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float*
     xf.sel_ntiles = tot + 1;
     {
         Plan plf;
-        if ((rc = prepare(c, plf, 1, p, &out, true, &xf, 1))) return rc;
+        if ((rc = prepare(c, plf, 1, p, &out, true, &xf, 2))) return rc;
         if (plf.jobs[0].v.W / plf.jobs[0].v.CPP != ppw) return fail(c, CLM_E_HIP, "fallback tile shape changed");
-        if ((rc = enqueue_job(c, plf, false))) return rc;
+        if ((rc = run_plan(c, plf, false, nullptr))) return rc;
     }
     if (d_flag) HIPCHK(c, hipMemcpyAsync(d_flag, c->w_flag.p, (size_t)P, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -974,6 +1070,8 @@ void clm_destroy(clm_ctx* c) {
     }
     for (TileCache& tc : c->tcache) release(tc.tiles);
     release(c->tune_tiles);
+    DevBuf* cb[] = {&c->cas_list, &c->cas_tiles, &c->cas_plist, &c->cas_cnt};
+    for (DevBuf* b : cb) release(*b);
     DevBuf* snb[] = {&c->sn_h1, &c->sn_h2, &c->sn_z, &c->sn_y, &c->sn_dy, &c->sn_dz, &c->sn_dz2, &c->sn_dz1};
     for (DevBuf* b : snb) release(*b);
     DevBuf* wb[] = {&c->w_cl1, &c->w_cl2, &c->w_j, &c->w_corr, &c->w_shift, &c->w_lsh, &c->w_flag, &c->w_rows,

@@ -164,10 +164,11 @@ def equation_rows(cl1, cl2, L):
 
 def estimate_shifts(rec, K: int, L: int, iters: int = 5, weights=None):
     """2-D shift per image (K x 2) from per-pair records by iteratively
-    reweighted least squares (Cauchy weights) on the shift equations; the
-    minimum-norm solution (LSQR from zero) fixes the 3-dim null space."""
+    reweighted least squares (Cauchy weights) on the shift equations.  Each
+    step solves the 2K x 2K normal equations by eigendecomposition and keeps
+    the minimum-norm solution: eigenvalues below 1e-9 of the largest (the
+    3-dim null space of projected 3-D translations) are dropped."""
     import scipy.sparse as sp
-    from scipy.sparse.linalg import lsqr
 
     cl1 = np.asarray(rec["cl1"])
     cl2 = np.asarray(rec["cl2"])
@@ -183,8 +184,12 @@ def estimate_shifts(rec, K: int, L: int, iters: int = 5, weights=None):
     w = np.ones(n) if weights is None else np.asarray(weights, np.float64)[ok]
     x = np.zeros(2 * K)
     for _ in range(iters):
-        sw = np.sqrt(w)
-        x = lsqr(sp.diags(sw) @ A, sw * rhs, atol=1e-10, btol=1e-10, iter_lim=2000)[0]
+        Aw = sp.diags(w) @ A
+        N = (A.T @ Aw).toarray()
+        g = A.T @ (w * rhs)
+        ev, V = np.linalg.eigh(N)
+        keep = ev > 1e-9 * ev[-1]
+        x = V[:, keep] @ ((V[:, keep].T @ g) / ev[keep])
         r = A @ x - rhs
         scale = 1.4826 * np.median(np.abs(r)) + 1e-6
         w = 1.0 / (1.0 + (r / (2.385 * scale)) ** 2)
@@ -318,3 +323,154 @@ def calibrate_gate(pf, prop, full_rec, max_shift=12.0, shift_step=0.5, window=1.
     idx = np.flatnonzero(ok)
     gate = float(corr[order[idx[-1]]]) if idx.size else float(corr.max())
     return gate, table
+
+
+# --------------------------------------------------------------- evaluation
+def pair_shift_error(prop, shifts, rots, L=T // 2, n_pairs=20000, seed=0):
+    """RMS error (px) of the pair 1-D shifts that proposals predict at the
+    true common lines, against the true shifts' values there.  Both use the
+    shift-equation coefficients (cryo_clmatrix_cpu.m:722-738), so a common
+    3-D translation of the proposals (their null space) does not count."""
+    from . import sim
+
+    prop = np.asarray(prop, np.float64)
+    K = prop.shape[0]
+    rng = np.random.default_rng(seed)
+    m = min(n_pairs, K * (K - 1) // 2)
+    k1 = rng.integers(0, K, size=3 * m)
+    k2 = rng.integers(0, K, size=3 * m)
+    ok = k1 < k2
+    k1, k2 = k1[ok][:m], k2[ok][:m]
+    err, base = [], []
+    for a, b in zip(k1, k2):
+        l1, l2 = sim.commonline_R(rots[:, :, a], rots[:, :, b], 2 * L)
+        c1, c2 = l1 + 1, l2 + 1
+        if c1 > L:  # the reference reports l1 in [1, L]: flip both lines
+            c1, c2 = c1 - L, (c2 + L - 1) % (2 * L) + 1
+        st = sim.true_shift_1d(c1, c2, L, shifts[a], shifts[b])
+        sp = sim.true_shift_1d(c1, c2, L, prop[a], prop[b])
+        err.append(sp - st)
+        base.append(st)
+    err, base = np.asarray(err), np.asarray(base)
+    return float(np.sqrt(np.mean(err ** 2))), float(np.sqrt(np.mean(base ** 2)))
+
+
+def evaluate(net: "ShiftNet", K=5000, seed=1005, snr=1 / 16, n=129, n_r=65, max_shift=12.0, shift_step=0.5,
+             window=1.0, gate=None, log=print):
+    """The learned form on one stack: full-grid search against the windowed
+    search with the network's proposals and with zero proposals (the
+    method without the network): time, agreement with the full grid,
+    detection against the true common lines, pair-shift prediction error."""
+    import torch
+
+    from . import sim
+    from .clmatrix import clmatrix_records
+
+    st = sim.make_stack(K=K, n=n, n_r=n_r, n_theta=T, snr=snr, max_shift_2d=max_shift, shift_kind="uniform",
+                        seed=seed, device="cuda")
+    gate = float(net.meta.get("gate", 0.0)) if gate is None else float(gate)
+    P = K * (K - 1) // 2
+    torch.cuda.synchronize()
+    # warm both paths once (kernel code, buffers), then time
+    clmatrix_records(st.pf, max_shift, shift_step, precision="bf16")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    full, fst = clmatrix_records(st.pf, max_shift, shift_step, precision="bf16")
+    torch.cuda.synchronize()
+    t_full = time.perf_counter() - t
+    full = {k: v.cpu().numpy() for k, v in full.items()}
+    prop = net.propose(st.pf)
+    clmatrix_windowed(st.pf, prop, max_shift, shift_step, window, gate)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    prop = net.propose(st.pf)
+    torch.cuda.synchronize()
+    t_net = time.perf_counter() - t
+    t = time.perf_counter()
+    rec, ws = clmatrix_windowed(st.pf, prop, max_shift, shift_step, window, gate)
+    torch.cuda.synchronize()
+    t_win = time.perf_counter() - t
+    rec = {k: v.cpu().numpy() for k, v in rec.items()}
+    zero = torch.zeros_like(prop)
+    rz, wz = clmatrix_windowed(st.pf, zero, max_shift, shift_step, window, gate)
+    rz = {k: v.cpu().numpy() for k, v in rz.items()}
+
+    def agree(r):
+        return float(np.mean((r["cl1"] == full["cl1"]) & (r["cl2"] == full["cl2"]) & (r["j"] == full["j"])))
+
+    nd = min(K, 1000)  # detection on the first images (all their pairs)
+    ref = sim.clmatrix_cheat(st.rots[:, :, :nd], 2 * (T // 2))
+
+    def detect(r):
+        cl = np.zeros((nd, nd))
+        k1, k2 = np.triu_indices(K, 1)
+        m = k2 < nd
+        cl[k1[m], k2[m]] = r["cl1"][m]
+        cl[k2[m], k1[m]] = r["cl2"][m]
+        return float(sim.comparecl(cl, ref, 2 * (T // 2), 5))
+
+    pe_net, pe_zero = pair_shift_error(prop.cpu().numpy(), st.shifts, st.rots)
+    out = dict(K=K, pairs=P, snr=snr, seed=seed, full_shifts=ws["full_shifts"], window_shifts=ws["window_shifts"],
+               gate=gate, full_seconds=t_full, full_pairs_per_s=P / t_full,
+               net_seconds=t_net, windowed_seconds=t_win, learned_pairs_per_s=P / (t_net + t_win),
+               speedup=t_full / (t_net + t_win),
+               settled=ws["accepted"] / P, below_gate=ws["below_gate"] / P, off_grid=ws["off_grid"] / P,
+               agree_full=agree(rec), agree_full_zero_proposals=agree(rz),
+               settled_zero_proposals=wz["accepted"] / P,
+               detect_5deg_full=detect(full), detect_5deg_learned=detect(rec), detect_5deg_zero=detect(rz),
+               pair_shift_rms_px_net=pe_net, pair_shift_rms_px_zero=pe_zero, window_stats=ws)
+    log(out)
+    return out
+
+
+# ---------------------------------------------------------------------- CLI
+def _cmd_train(args):
+    import json
+
+    t0 = time.time()
+    x, y, truth, errs = make_training_set(n_stacks=args.stacks, K=args.K, seed0=args.seed0)
+    t_data = time.time() - t0
+    net = ShiftNet(seed=args.init_seed)
+    hist = train(net, x, y, epochs=args.epochs, batch=args.batch, lr=args.lr)
+    t_train = time.time() - t0 - t_data
+    info = dict(stacks=args.stacks, K=args.K, seed0=args.seed0, input_snrs=[1 / 16, 1 / 4, 1.0],
+                epochs=args.epochs, batch=args.batch, lr=args.lr, samples=int(x.shape[0]),
+                ls_target_rms_px=float(np.mean(errs)), final_loss=hist[-1][0], val_rms_px=hist[-1][1],
+                seconds_data=t_data, seconds_train=t_train, steps=net.steps)
+    net.save(args.out, training=info, window=1.0)
+    print(json.dumps(info))
+
+
+def main(argv=None):
+    import argparse
+
+    ap = argparse.ArgumentParser(prog="python -m aspire_amd.shiftnet",
+                                 description="Train the shift-proposal network with libclm's kernels.")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--stacks", type=int, default=20)
+    t.add_argument("--K", type=int, default=1000)
+    t.add_argument("--seed0", type=int, default=5001)
+    t.add_argument("--init-seed", type=int, default=0)
+    t.add_argument("--epochs", type=int, default=30)
+    t.add_argument("--batch", type=int, default=256)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--out", default=WEIGHTS)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--K", type=int, default=5000)
+    e.add_argument("--snr", type=float, default=1 / 16)
+    e.add_argument("--seed", type=int, default=1005)
+    e.add_argument("--gate", type=float, default=None)
+    args = ap.parse_args(argv)
+    if args.cmd == "train":
+        _cmd_train(args)
+    elif args.cmd == "evaluate":
+        import json
+
+        net = ShiftNet.load(args.weights)
+        print(json.dumps(evaluate(net, K=args.K, snr=args.snr, seed=args.seed, gate=args.gate, log=lambda *a: None)))
+
+
+if __name__ == "__main__":
+    main()

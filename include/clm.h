@@ -67,7 +67,9 @@ typedef struct clm_params {
     double tie_margin;   /* FP64 re-score of candidates within this of the
                             FP32 best (correlation units), default 1e-3       */
     int32_t variant;     /* 0 = tuned/default kernel variant, else a variant id */
-    int32_t reserved;    /* tuning bits; bit 0 disables the lagged wave schedule */
+    int32_t reserved;    /* tuning bits: bit 0 disables the lagged wave schedule;
+                            bit 1 re-runs every uncertain pair with FP32
+                            operands (default: only when >= 2048 are)       */
 } clm_params;
 
 typedef struct clm_stats {
@@ -84,6 +86,8 @@ typedef struct clm_stats {
     double ms_total;           /* whole device job                                */
     int32_t variant;           /* kernel variant that ran                         */
     int32_t n_eval_shifts;     /* shifts the contraction evaluated                */
+    int64_t refined_pairs;     /* uncertain pairs re-run with FP32 operands
+                                  (single-stack calls, >= 2048 uncertain pairs) */
 } clm_stats;
 
 /* Library version (CLM_VERSION of the build). */
