@@ -185,6 +185,7 @@ struct StackJob {
     int group = -1;
     const double* lsh = nullptr;     // per-line pre-shift [K][L] (windowed search) or null
     int force64 = 0;                 // FP64 near-tie re-score even for FP32 operands (refinement)
+    int twin_src = -1;               // FP32 twin: table index of the stack it refines
     // selected pairs only (device-built): list of (stack, slot), count, pass-1 tiles
     const SlotRef* sel_list = nullptr;
     const int* sel_count = nullptr;
@@ -197,6 +198,9 @@ struct StackJob {
 struct Group {
     int opk = 0, exact = 0, ct = 0, Rp = 0, sym = 0, W = 0, CPP = 0, lag = 0;
     int sel = 0;                // one stack of selected pairs (device-built tiles and list)
+    int norefine = 0;           // no FP32 twin: uncertain pairs take the second operand-type pass
+    int twin = -1;              // index of the FP32 twin group refining this group's uncertain pairs
+    int twin_of = -1;           // (twin groups) the group it refines
     std::vector<int> members;   // job indices
     int first = 0;              // first table index
     long long P = 0;            // pairs of all members
@@ -213,6 +217,7 @@ struct Plan {
     std::vector<int> order;      // table index -> job index
     std::vector<StackDev> table;
     int sb_base = 0;             // first per-stack buffer set of the context used
+    long long tw_max = 0;        // largest group refined by a twin (pairs)
     long long P = 0;
     int n_unc_cap = 0;           // entries of the shared pass-2 buffers
 };
@@ -266,7 +271,7 @@ int plan_stack(clm_ctx* c, const clm_params* p, StackJob& j) {
 // Allocate and upload everything the device job reads; build the table.
 // Nothing here is captured: the job itself (enqueue_job) only launches kernels.
 int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_io* ios, bool build_tiles = true,
-            const StackJob* extra = nullptr, int sb_base = 0) {
+            const StackJob* extra = nullptr, int sb_base = 0, bool twins = true) {
     int rc;
     pl.jobs.resize(n);
     pl.sb_base = sb_base;
@@ -287,12 +292,13 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         if (j.P == 0) continue;
         const Geom& g = j.s.g;
         const int lag = j.lag;
+        const int norefine = (j.p.reserved & 2) ? 1 : 0;
         int gi = -1;
         for (size_t q = 0; q < pl.groups.size() && !j.sel_list; q++) {
             const Group& G = pl.groups[q];
-            if (!G.sel && G.exact == j.exact && (j.exact || (G.opk == j.opk && G.ct == j.v.ct && G.Rp == g.Rp &&
-                                                   G.sym == g.sym && G.W == j.v.W && G.CPP == j.v.CPP &&
-                                                   G.lag == lag))) {
+            if (!G.sel && G.exact == j.exact && G.norefine == norefine &&
+                (j.exact || (G.opk == j.opk && G.ct == j.v.ct && G.Rp == g.Rp && G.sym == g.sym && G.W == j.v.W &&
+                             G.CPP == j.v.CPP && G.lag == lag))) {
                 gi = (int)q;
                 break;
             }
@@ -302,27 +308,66 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
             G.opk = j.opk; G.exact = j.exact; G.ct = j.v.ct; G.Rp = g.Rp; G.sym = g.sym;
             G.W = j.v.W; G.CPP = j.v.CPP; G.lag = lag;
             G.sel = j.sel_list != nullptr;
+            G.norefine = norefine;
             pl.groups.push_back(G);
             gi = (int)pl.groups.size() - 1;
         }
         pl.groups[gi].members.push_back(i);
         j.group = gi;
     }
-    // table in group order; pair_base relative to the group
+    // FP32 twins: for every BF16/FP16 group, FP32 copies of its stacks (same
+    // rays, records and outputs; FP32 operand planes and A-stacks) that re-run
+    // the group's uncertain pairs with a 16x tighter error bound.  A broad
+    // correlation peak (clean, high-SNR data) leaves many candidates inside
+    // the BF16 bound, and this keeps such pairs off the FP64 exhaustive path.
+    const size_t n_orig_groups = pl.groups.size();
+    for (size_t gi = 0; gi < n_orig_groups && twins; gi++) {
+        if (pl.groups[gi].exact || pl.groups[gi].opk == OPK_F32 || pl.groups[gi].norefine) continue;
+        const StackJob& j0 = pl.jobs[pl.groups[gi].members[0]];
+        const Variant v32 = choose_variant(OPK_F32, (j0.s.g.L + 31) / 32, j0.s.g.Rp, 0, 0);
+        if (!v32.ct) continue;
+        Group T;
+        T.opk = OPK_F32; T.ct = v32.ct; T.Rp = pl.groups[gi].Rp; T.sym = pl.groups[gi].sym;
+        T.W = v32.W; T.CPP = v32.CPP; T.lag = (v32.W % 2 == 0) ? 1 : 0;
+        T.sel = 1;
+        T.twin_of = (int)gi;
+        for (int i : pl.groups[gi].members) {
+            StackJob t = pl.jobs[i];
+            t.opk = OPK_F32;
+            t.v = v32;
+            t.s.g.NCT = v32.NCTp;
+            t.s.g.Lp = 32 * v32.NCTp;
+            t.force64 = t.use64;
+            t.twin_src = i;  // job index for now; table index below
+            t.group = (int)pl.groups.size();
+            pl.jobs.push_back(t);
+            T.members.push_back((int)pl.jobs.size() - 1);
+        }
+        pl.groups[gi].twin = (int)pl.groups.size();
+        pl.groups.push_back(T);
+    }
+    // table in group order (twins last); pair_base relative to the group
     pl.order.clear();
     pl.P = 0;
     long long ubase = 0;
+    std::vector<int> table_of(pl.jobs.size(), -1);
     for (Group& G : pl.groups) {
         G.first = (int)pl.order.size();
         G.P = 0;
         for (int i : G.members) {
+            table_of[i] = (int)pl.order.size();
             pl.order.push_back(i);
             G.P += pl.jobs[i].P;
         }
         G.ubase = ubase;
         if (!G.exact) ubase += G.P;
-        pl.P += G.P;
+        if (G.twin_of < 0) pl.P += G.P;
     }
+    pl.tw_max = 0;
+    for (StackJob& j : pl.jobs)
+        if (j.twin_src >= 0) j.twin_src = table_of[j.twin_src];
+    for (const Group& G : pl.groups)
+        if (G.twin_of >= 0) pl.tw_max = std::max(pl.tw_max, G.P);
     const int nt = (int)pl.order.size();
     if (c->sb.size() < (size_t)(sb_base + nt)) c->sb.resize(sb_base + nt);
     pl.table.assign(nt, StackDev{});
@@ -332,6 +377,22 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         const Geom& g = j.s.g;
         const long long K = g.K;
         const size_t osz = j.opk == OPK_F32 ? 4 : 2;
+        if (j.twin_src >= 0) {
+            // FP32 twin: own operand planes and A-stacks, everything else shared
+            if ((rc = ensure(c, B.Bop, (size_t)K * 2 * g.Lp * g.Rp * 4))) return rc;
+            if ((rc = ensure(c, B.astk, (size_t)K * g.NAT * (32 * 2 * (size_t)g.Rp * 4)))) return rc;
+            StackDev& d = pl.table[ti];
+            d = pl.table[j.twin_src];
+            d.g = g;
+            d.astk = B.astk.p;
+            d.Bop = B.Bop.p;
+            d.coef = (2.0f * 0x1p-24f + 0x1p-48f + (2.0f * g.Rp + 16.0f) * 0x1p-24f) * 1.01f;
+            d.sub = 0.0f;
+            d.tau_t = j.force64 ? (float)(0.5 * j.p.tie_margin) : 0.0f;
+            d.fp64 = j.force64;
+            d.CPP = j.v.CPP;
+            continue;
+        }
         if ((rc = ensure(c, B.P64, (size_t)K * g.L * g.R * sizeof(double2)))) return rc;
         if ((rc = ensure(c, B.P32, (size_t)K * g.L * g.Rp * sizeof(float2)))) return rc;
         if ((rc = ensure(c, B.Bop, (size_t)K * 2 * g.Lp * g.Rp * osz))) return rc;
@@ -385,6 +446,8 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
     if ((rc = ensure(c, c->erecs, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(EmitRec)))) return rc;
     if ((rc = ensure(c, c->plist, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(int4)))) return rc;
     if ((rc = ensure(c, c->tiles2, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(int4)))) return rc;
+    if ((rc = ensure(c, c->tw_tiles, (size_t)std::max<long long>(pl.tw_max, 1) * sizeof(int4)))) return rc;
+    if ((rc = ensure(c, c->tw_plist, (size_t)std::max<long long>(pl.tw_max, 1) * sizeof(int4)))) return rc;
     // kernels and pass-1 tiles per group
     for (Group& G : pl.groups) {
         if (G.exact) continue;
@@ -453,6 +516,20 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
         const StackJob& j = pl.jobs[pl.order[ti]];
         const StackBufs& B = c->sb[pl.sb_base + ti];
         const Geom& g = j.s.g;
+        if (j.twin_src >= 0) {
+            const StackDev& src = pl.table[j.twin_src];
+            const long long nb32 = (long long)g.K * 2 * g.Lp * g.Rp;
+            hipLaunchKernelGGL(bop32_kernel, dim3((unsigned)((nb32 + 255) / 256)), dim3(256), 0, c->stream, g,
+                               src.P32, (float*)B.Bop.p);
+            HIPCHK(c, hipGetLastError());
+            const long long nch = (long long)g.K * g.NAT * (long long)(32 * 2 * (size_t)g.Rp * 4 / 16);
+            AstkFn af = pick_astk(OPK_F32, g.Rp, g.sym != 0);
+            if (!af) return fail(c, CLM_E_UNSUP, "no FP32 A-stack kernel for Rp=%d", g.Rp);
+            hipLaunchKernelGGL(af, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, c->stream, g, src.P32,
+                               src.ph32, (int4*)B.astk.p);
+            HIPCHK(c, hipGetLastError());
+            continue;
+        }
         if ((rc = enqueue_fill(c, B.nrm.p, g.K, 0))) return rc;
         const long long lines = (long long)g.K * g.Lp;
         const unsigned nb = (unsigned)((lines + 255) / 256);
@@ -495,9 +572,10 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
     if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = enqueue_prep(c, pl))) return rc;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    // pass 1, one persistent launch per group
+    // pass 1, one persistent launch per group (twins run in phase B)
     for (size_t gi = 0; gi < pl.groups.size(); gi++) {
         const Group& G = pl.groups[gi];
+        if (G.twin_of >= 0) continue;
         if (G.exact) {
             Exact64Args ea{};
             ea.stacks = dtab;
@@ -542,10 +620,10 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
         HIPCHK(c, hipGetLastError());
     }
     if (timed) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    // re-score; uncertain pairs are listed for the exact-threshold second pass
+    // re-score; uncertain pairs are listed for the refinement or second pass
     for (size_t gi = 0; gi < pl.groups.size(); gi++) {
         const Group& G = pl.groups[gi];
-        if (G.exact) continue;
+        if (G.exact || G.twin_of >= 0) continue;
         unsigned long long* cg = ctr + 16 + 4 * gi;
         SlotRef* ul = (SlotRef*)c->ulist.p + G.ubase;
         RescoreArgs ra{};
@@ -574,9 +652,57 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
     }
     }  // phase A
     if (!(phases & PHASE_B)) return CLM_OK;
-    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+    for (size_t gi0 = 0; gi0 < pl.groups.size(); gi0++) {
+        if (pl.groups[gi0].exact || pl.groups[gi0].twin_of >= 0) continue;
+        size_t gi = gi0;  // the group whose uncertain list takes the second pass
+        if (pl.groups[gi0].twin >= 0) {
+            // refinement: the FP32 twin re-runs the uncertain pairs (pass 1 over
+            // one-pair tiles, re-score of its records); its own uncertain pairs
+            // then take the FP32 second pass below
+            const Group& G0 = pl.groups[gi0];
+            const Group& T = pl.groups[G0.twin];
+            unsigned long long* c0 = ctr + 16 + 4 * gi0;
+            unsigned long long* cT = ctr + 16 + 4 * (size_t)G0.twin;
+            const SlotRef* ul0 = (const SlotRef*)c->ulist.p + G0.ubase;
+            const int shift = T.first - G0.first;
+            hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((G0.P + 255) / 256, 1024)),
+                               dim3(256), 0, c->stream, dtab, (const int*)(c0 + 1), ul0, (int)G0.P,
+                               (int4*)c->tw_tiles.p, (int4*)c->tw_plist.p, (int*)(cT + 3), (int*)(ctr + 4),
+                               (SlotRef*)c->fblist.p, shift);
+            HIPCHK(c, hipGetLastError());
+            PairArgs pa{};
+            pa.stacks = dtab;
+            pa.tiles = (const int4*)c->tw_tiles.p;
+            pa.ntiles = 0;
+            pa.ntiles_dev = (const int*)(cT + 3);
+            pa.counter = (int*)(cT + 0);
+            pa.plist = (const int4*)c->tw_plist.p;
+            pa.erecs = nullptr;
+            pa.CPP = T.CPP;
+            pa.lag = T.lag;
+            const unsigned g1 = (unsigned)std::min<long long>((long long)c->n_cu * T.occ, G0.P);
+            hipLaunchKernelGGL(T.fn, dim3(g1), dim3(64 * T.W), T.lds, c->stream, pa);
+            HIPCHK(c, hipGetLastError());
+            RescoreArgs rt{};
+            rt.stacks = dtab;
+            rt.mode = 2;
+            rt.n_items_dev = (const int*)(c0 + 1);
+            rt.item_cap = (int)G0.P;
+            rt.ulist = ul0;
+            rt.stack_shift = shift;
+            rt.unc_count = (int*)(cT + 1);
+            rt.unc_list = (SlotRef*)c->ulist.p + T.ubase;
+            rt.fb_count = (int*)(ctr + 4);
+            rt.fb_list = (SlotRef*)c->fblist.p;
+            rt.n64 = ctr + 1;
+            rt.ncell = ctr + 6;
+            rt.nemit = ctr + 7;
+            const unsigned rg = (unsigned)std::min<long long>((G0.P + 3) / 4, (long long)c->n_cu * 16);
+            hipLaunchKernelGGL(rescore_kernel, dim3(rg), dim3(256), 0, c->stream, rt);
+            HIPCHK(c, hipGetLastError());
+            gi = (size_t)G0.twin;
+        }
         const Group& G = pl.groups[gi];
-        if (G.exact) continue;
         unsigned long long* cg = ctr + 16 + 4 * gi;
         SlotRef* ul = (SlotRef*)c->ulist.p + G.ubase;
         RescoreArgs ra{};
@@ -592,7 +718,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
         hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((G.P + 255) / 256, 1024)),
                            dim3(256), 0, c->stream, dtab, (const int*)(cg + 1), (const SlotRef*)ul, cap,
                            (int4*)c->tiles2.p, (int4*)c->plist.p, (int*)(cg + 3), (int*)(ctr + 4),
-                           (SlotRef*)c->fblist.p);
+                           (SlotRef*)c->fblist.p, 0);
         HIPCHK(c, hipGetLastError());
         PairArgs pa{};
         pa.stacks = dtab;
@@ -643,8 +769,12 @@ int add_stats(clm_ctx* c, const Plan& pl, clm_stats* st) {
     st->emitted += (int64_t)h[7];
     st->fallback_pairs += as_int(h[4]);
     for (size_t gi = 0; gi < pl.groups.size(); gi++) {
-        if (pl.groups[gi].exact && !pl.groups[gi].sel) st->fallback_pairs += pl.groups[gi].P;
-        else if (!pl.groups[gi].exact) st->uncertain_pairs += as_int(h[16 + 4 * gi + 1]);
+        const Group& G = pl.groups[gi];
+        if (G.exact && !G.sel) st->fallback_pairs += G.P;
+        else if (!G.exact && G.twin_of < 0) {
+            st->uncertain_pairs += as_int(h[16 + 4 * gi + 1]);
+            if (G.twin >= 0) st->refined_pairs += as_int(h[16 + 4 * gi + 1]);
+        }
     }
     return CLM_OK;
 }
@@ -652,7 +782,8 @@ int add_stats(clm_ctx* c, const Plan& pl, clm_stats* st) {
 void begin_stats(const Plan& pl, clm_stats* st) {
     if (!st) return;
     std::memset(st, 0, sizeof *st);
-    for (const StackJob& j : pl.jobs) st->pairs += j.P;
+    for (const StackJob& j : pl.jobs)
+        if (j.twin_src < 0) st->pairs += j.P;
     if (!pl.jobs.empty()) {
         st->n_eval_shifts = pl.jobs[0].s.g.NE;
         st->variant = pl.jobs[0].v.ct * 100 + pl.jobs[0].v.W;
@@ -667,72 +798,12 @@ void timed_stats(clm_ctx* c, clm_stats* st) {
     st->ms_total = event_ms(c->ev[0], c->ev[3]);
 }
 
-// Uncertain pairs of a single-stack plan beyond which the pairs are re-run
-// with FP32 operands instead of the second BF16 pass: a broad correlation
-// peak (clean, high-SNR data) puts many candidates inside the BF16 error
-// bound, which the FP32 bound (16x tighter) cuts down.
-constexpr int REFINE_MIN = 2048;
-
-// Run a plan to completion (no graph).  Single-stack MFMA16 plans whose
-// first pass leaves many pairs uncertain continue with the refinement: the
-// uncertain list is copied aside, and a plan over those pairs only, with
-// FP32 operands (FP64 near-tie re-score kept), settles them into the same
-// outputs.
-int run_plan(clm_ctx* c, const Plan& pl, bool timed, clm_stats* st, int depth = 0) {
+// Run a plan to completion (no graph).
+int run_plan(clm_ctx* c, const Plan& pl, bool timed, clm_stats* st) {
     int rc;
-    const bool can = pl.groups.size() == 1 && !pl.groups[0].exact && pl.jobs.size() == 1 &&
-                     pl.jobs[0].opk != OPK_F32 && depth == 0;
-    if (!can) {
-        if ((rc = enqueue_job(c, pl, timed))) return rc;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        return add_stats(c, pl, st);
-    }
-    if ((rc = enqueue_job(c, pl, timed, PHASE_A))) return rc;
-    unsigned long long* ctr = (unsigned long long*)c->ctr.p;
-    int n_unc = 0;
-    HIPCHK(c, hipMemcpyAsync(&n_unc, ctr + 16 + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = enqueue_job(c, pl, timed))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const bool force = (pl.jobs[0].p.reserved & 2) != 0;  // refine any uncertain pair
-    if (n_unc == 0 || (n_unc < REFINE_MIN && !force)) {
-        if ((rc = enqueue_job(c, pl, timed, PHASE_B))) return rc;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        return add_stats(c, pl, st);
-    }
-    const StackJob& j0 = pl.jobs[0];
-    const long long P = j0.P;
-    if ((rc = ensure(c, c->cas_list, (size_t)P * sizeof(SlotRef))) || (rc = ensure(c, c->cas_tiles, (size_t)P * 16)) ||
-        (rc = ensure(c, c->cas_plist, (size_t)P * 16)) || (rc = ensure(c, c->cas_cnt, 16)))
-        return rc;
-    HIPCHK(c, hipMemcpyAsync(c->cas_list.p, c->ulist.p, (size_t)n_unc * sizeof(SlotRef), hipMemcpyDeviceToDevice,
-                             c->stream));
-    const int cnt[4] = {n_unc, 0, 0, 0};
-    HIPCHK(c, hipMemcpyAsync(c->cas_cnt.p, cnt, sizeof cnt, hipMemcpyHostToDevice, c->stream));
-    // one pair per tile (the FP32 variants of long rays hold one pair per workgroup)
-    hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((n_unc + 255) / 256, 1024)), dim3(256),
-                       0, c->stream, (const StackDev*)c->table.p, (const int*)c->cas_cnt.p,
-                       (const SlotRef*)c->cas_list.p, n_unc, (int4*)c->cas_tiles.p, (int4*)c->cas_plist.p,
-                       (int*)c->cas_cnt.p + 1, (int*)c->cas_cnt.p + 2, (SlotRef*)c->cas_list.p);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if ((rc = add_stats(c, pl, st))) return rc;
-    clm_params q = j0.p;
-    q.precision = CLM_FP32;
-    q.variant = 0;
-    StackJob x;
-    x.lsh = j0.lsh;
-    x.force64 = j0.use64;
-    x.sel_list = (const SlotRef*)c->cas_list.p;
-    x.sel_count = (const int*)c->cas_cnt.p;
-    x.sel_tiles = (const int4*)c->cas_tiles.p;
-    x.sel_plist = (const int4*)c->cas_plist.p;
-    x.sel_ntiles = (const int*)c->cas_cnt.p + 1;
-    Plan pr;
-    if ((rc = prepare(c, pr, 1, &q, &j0.io, true, &x, pl.sb_base + 1))) return rc;
-    if ((rc = enqueue_job(c, pr, false))) return rc;
-    if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (st) st->refined_pairs += n_unc;
-    return add_stats(c, pr, st);
+    return add_stats(c, pl, st);
 }
 
 void drop_graph(clm_ctx* c) {
@@ -762,7 +833,8 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
         const void* sp[] = {B.P64.p, B.P32.p, B.Bop.p, B.nrm.p, B.astk.p, B.recs.p, B.ph32.p, B.ph64.p, B.shiftv.p, B.H.p};
         add(sp, sizeof sp);
     }
-    const void* bufs[] = {c->table.p, c->ctr.p, c->ulist.p, c->erecs.p, c->plist.p, c->tiles2.p, c->fblist.p};
+    const void* bufs[] = {c->table.p, c->ctr.p, c->ulist.p, c->erecs.p, c->plist.p, c->tiles2.p, c->fblist.p,
+                          c->tw_tiles.p, c->tw_plist.p};
     add(bufs, sizeof bufs);
     for (const Group& G : pl.groups) {
         const void* t = G.tc ? G.tc->tiles.p : nullptr;
@@ -958,7 +1030,7 @@ int autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int sample_pairs
             q.reserved = (p->reserved & ~1) | (lag ? 0 : 1);
             const clm_stack_io io{d_pf, nullptr, nullptr, nullptr, nullptr, nullptr};
             Plan pl;
-            if ((rc = prepare(c, pl, 1, &q, &io, false))) return rc;
+            if ((rc = prepare(c, pl, 1, &q, &io, false, nullptr, 0, false))) return rc;
             const Group& G = pl.groups[0];
             const long long K = g0.K;
             const int PPW = G.W / G.CPP;
@@ -1070,8 +1142,8 @@ void clm_destroy(clm_ctx* c) {
     }
     for (TileCache& tc : c->tcache) release(tc.tiles);
     release(c->tune_tiles);
-    DevBuf* cb[] = {&c->cas_list, &c->cas_tiles, &c->cas_plist, &c->cas_cnt};
-    for (DevBuf* b : cb) release(*b);
+    release(c->tw_tiles);
+    release(c->tw_plist);
     DevBuf* snb[] = {&c->sn_h1, &c->sn_h2, &c->sn_z, &c->sn_y, &c->sn_dy, &c->sn_dz, &c->sn_dz2, &c->sn_dz1};
     for (DevBuf* b : snb) release(*b);
     DevBuf* wb[] = {&c->w_cl1, &c->w_cl2, &c->w_j, &c->w_corr, &c->w_shift, &c->w_lsh, &c->w_flag, &c->w_rows,

@@ -46,8 +46,8 @@ struct clm_ctx {
     // windowed search: window records, per-line shifts, flags, selected-pair lists
     clmh::DevBuf w_cl1, w_cl2, w_j, w_corr, w_shift, w_lsh, w_flag, w_rows, w_list, w_plist, w_tiles, w_counts,
         w_shiftv;
-    // refinement of uncertain pairs with FP32 operands: list, tiles, pairs, counts
-    clmh::DevBuf cas_list, cas_tiles, cas_plist, cas_cnt;
+    // refinement of uncertain pairs by the FP32 twins: one-pair tiles and pairs
+    clmh::DevBuf tw_tiles, tw_plist;
     // shift network: activations and gradients of the last batch
     clmh::DevBuf sn_h1, sn_h2, sn_z, sn_y, sn_dy, sn_dz, sn_dz2, sn_dz1;
     // autotuned variants: shape key -> (ct, W, lag)

@@ -115,6 +115,7 @@ struct RescoreArgs {
     const int* n_items_dev;   // modes 1, 2: number of list entries (device)
     int item_cap;             // modes 1, 2: entries beyond this are not processed
     const SlotRef* ulist;     // modes 1, 2: u -> (stack, slot)
+    int stack_shift;          // modes 1, 2: added to the listed stack index
     const EmitRec* erecs;     // mode 1: candidates of entry u
     int* unc_count;           // mode 0: uncertain pairs appended here
     SlotRef* unc_list;
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             stk = a.stack0 + (a.nstacks > 1 ? find_stack(a.stacks + a.stack0, a.nstacks, item) : 0);
             slot = (int)(item - a.stacks[stk].pair_base);
         } else {
-            stk = a.ulist[item].stack;
+            stk = a.ulist[item].stack + a.stack_shift;
             slot = a.ulist[item].slot;
         }
         const StackDev& sd = a.stacks[stk];
@@ -358,11 +359,14 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
     }
 }
 
-// Pass-2 tiles from the uncertain list, on the device: one pair per tile
-// (entry u -> tile u and explicit pair u).  Entries past `cap` have no
-// pass-2 buffers and go straight to the FP64 exhaustive list.
+// Pass-2 (or refinement) tiles from an uncertain list, on the device: one
+// pair per tile (entry u -> tile u and explicit pair u); the tiles name stack
+// r.stack + stack_shift (the FP32 twin of r.stack for the refinement).
+// Entries past `cap` have no pass-2 buffers and go straight to the FP64
+// exhaustive list.
 __global__ void build_pass2_kernel(const StackDev* stacks, const int* n_unc, const SlotRef* ulist, int cap,
-                                   int4* tiles, int4* plist, int* ntiles, int* fb_count, SlotRef* fb_list) {
+                                   int4* tiles, int4* plist, int* ntiles, int* fb_count, SlotRef* fb_list,
+                                   int stack_shift) {
     const int n = *n_unc;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *ntiles = min(n, cap);
@@ -370,7 +374,7 @@ __global__ void build_pass2_kernel(const StackDev* stacks, const int* n_unc, con
         const SlotRef r = ulist[u];
         if (u < cap) {
             const PairRec& rec = stacks[r.stack].recs[r.slot];
-            tiles[u] = make_int4(rec.k1, u, 1 | (r.stack << 4), -1);
+            tiles[u] = make_int4(rec.k1, u, 1 | ((r.stack + stack_shift) << 4), -1);
             plist[u] = make_int4(rec.k2, r.slot, u, 0);
         } else {
             const int f = atomicAdd(fb_count, 1);
@@ -445,6 +449,24 @@ __global__ __launch_bounds__(256) void exact64_kernel(Exact64Args a) {
         if (threadIdx.x == 0) write_result(sd, slot, s_v[0], s_k[0]);
         __syncthreads();
     }
+}
+
+// FP32 operand planes [K][2][Lp][Rp] of a stack from its FP32 half lines
+// (the values prep_kernel writes for FP32 operands; zero padding).
+__global__ void bop32_kernel(Geom g, const float2* __restrict__ P32, float* __restrict__ Bop) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)g.K * 2 * g.Lp * g.Rp;
+    if (i >= n) return;
+    const int r = (int)(i % g.Rp);
+    const int l = (int)((i / g.Rp) % g.Lp);
+    const int plane = (int)((i / ((long long)g.Rp * g.Lp)) % 2);
+    const long long k = i / ((long long)g.Rp * g.Lp * 2);
+    float v = 0.f;
+    if (l < g.L) {
+        const float2 z = P32[((size_t)k * g.L + l) * g.Rp + r];
+        v = plane ? z.y : z.x;
+    }
+    Bop[i] = v;
 }
 
 __global__ void fill_i32(int* p, int n, int v) {

@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--sample-pairs", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-form line (windowed search with the shift network)")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic variant)")
     ap.add_argument("--tune-pairs", type=int, default=200000)
@@ -179,6 +181,21 @@ def main():
     log(f"parity sample {ns}: agree {same.mean():.5f}, near-tie disagreements {near}/{(~same).sum()}, "
         f"corr rel {corr_rel:.2e}, oracle {cpu_sec:.1f}s on {nth} threads")
 
+    # learned form on its own workload: K=5000, shifts U[-12,12], full grid
+    # max_shift 12 step 0.5 (49 shifts) against the windowed search (5 shifts)
+    # with the committed shift network
+    learned = None
+    from aspire_amd import shiftnet
+
+    if not args.no_learned and os.path.exists(shiftnet.WEIGHTS):
+        del st
+        torch.cuda.empty_cache()
+        net = shiftnet.ShiftNet.load()
+        learned = shiftnet.evaluate(net, K=5000, seed=1005, snr=1 / 16, log=lambda *a: None)
+        learned["workload"] = ("K=5000 n=129 n_r=65 n_theta=360 shifts U[-12,12] SNR=1/16; full grid "
+                               "max_shift=12 step=0.5 (49 shifts) vs window +-1 px (5 shifts) + exhaustive fallback")
+        log(f"learned: {learned}")
+
     cpu = None
     if not args.no_cpu:
         # 1-thread oracle on a bounded sample for the per-core figure
@@ -215,6 +232,7 @@ def main():
         "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "ms_pairs": st32["ms_pairs"],
                      "variant": st32["variant"], "autotune": tune32},
         "autotune": tune,
+        "learned": learned,
         "speedup_vs_fp32": value / fp32_value,
         "clstack_match_pct": 100.0 * float(same.mean()),
         "parity_sample": {"pairs": int(ns), "disagree": int((~same).sum()), "near_tie": int(near),

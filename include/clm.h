@@ -68,8 +68,9 @@ typedef struct clm_params {
                             FP32 best (correlation units), default 1e-3       */
     int32_t variant;     /* 0 = tuned/default kernel variant, else a variant id */
     int32_t reserved;    /* tuning bits: bit 0 disables the lagged wave schedule;
-                            bit 1 re-runs every uncertain pair with FP32
-                            operands (default: only when >= 2048 are)       */
+                            bit 1 disables the FP32 refinement of uncertain
+                            BF16/FP16 pairs (they take a second BF16/FP16
+                            pass instead)                                   */
 } clm_params;
 
 typedef struct clm_stats {
@@ -86,8 +87,8 @@ typedef struct clm_stats {
     double ms_total;           /* whole device job                                */
     int32_t variant;           /* kernel variant that ran                         */
     int32_t n_eval_shifts;     /* shifts the contraction evaluated                */
-    int64_t refined_pairs;     /* uncertain pairs re-run with FP32 operands
-                                  (single-stack calls, >= 2048 uncertain pairs) */
+    int64_t refined_pairs;     /* uncertain BF16/FP16 pairs re-run with FP32
+                                  operands                                      */
 } clm_stats;
 
 /* Library version (CLM_VERSION of the build). */
