@@ -173,6 +173,12 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 // and call; uncertain pairs beyond it go to the FP64 exhaustive kernel.
 constexpr int UCAP = 32768;
 
+// FP64 exhaustive search of items (pairs): the first EX_CAP items split their
+// shift grid over up to EX_SJ workgroups each (few items still fill the
+// GPU), the rest take one workgroup per item.
+constexpr long long EX_CAP = 4096;
+constexpr int EX_SJ = 64;
+
 // One stack of a call (one cryo_clmatrix problem).
 struct StackJob {
     clm_params p;
@@ -447,6 +453,8 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
     if ((rc = ensure(c, c->plist, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(int4)))) return rc;
     if ((rc = ensure(c, c->tiles2, (size_t)std::max(pl.n_unc_cap, 1) * sizeof(int4)))) return rc;
     if ((rc = ensure(c, c->tw_tiles, (size_t)std::max<long long>(pl.tw_max, 1) * sizeof(int4)))) return rc;
+    if ((rc = ensure(c, c->ex_v, (size_t)EX_CAP * EX_SJ * sizeof(double)))) return rc;
+    if ((rc = ensure(c, c->ex_k, (size_t)EX_CAP * EX_SJ * sizeof(long long)))) return rc;
     if ((rc = ensure(c, c->tw_plist, (size_t)std::max<long long>(pl.tw_max, 1) * sizeof(int4)))) return rc;
     // kernels and pass-1 tiles per group
     for (Group& G : pl.groups) {
@@ -560,6 +568,35 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
 // preprocessing, pair contraction and re-scoring.
 enum { PHASE_A = 1, PHASE_B = 2, PHASE_ALL = 3 };
 
+
+int enqueue_exact(clm_ctx* c, Exact64Args ea, long long n_upper, int S) {
+    if (n_upper <= 0) return CLM_OK;
+    const int sj = std::max(1, std::min(S, EX_SJ));
+    const long long cap = std::min<long long>(n_upper, EX_CAP);
+    ea.item0 = 0;
+    ea.item_cap = cap;
+    ea.sj = sj;
+    ea.sv = (double*)c->ex_v.p;
+    ea.sk = (long long*)c->ex_k.p;
+    hipLaunchKernelGGL(exact64_kernel, dim3((unsigned)std::min<long long>(cap, 256), (unsigned)sj), dim3(256), 0,
+                       c->stream, ea);
+    HIPCHK(c, hipGetLastError());
+    if (sj > 1) {
+        hipLaunchKernelGGL(exact64_reduce_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, c->stream, ea);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (n_upper > EX_CAP) {
+        ea.item0 = EX_CAP;
+        ea.item_cap = n_upper - EX_CAP;
+        ea.sj = 1;
+        hipLaunchKernelGGL(exact64_kernel,
+                           dim3((unsigned)std::min<long long>(n_upper - EX_CAP, (long long)c->n_cu * 8), 1), dim3(256),
+                           0, c->stream, ea);
+        HIPCHK(c, hipGetLastError());
+    }
+    return CLM_OK;
+}
+
 int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) {
     int rc;
     unsigned long long* ctr = (unsigned long long*)c->ctr.p;
@@ -588,9 +625,9 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
                 ea.list = pl.jobs[G.members[0]].sel_list;
                 ea.n_dev = pl.jobs[G.members[0]].sel_count;
             }
-            const unsigned nb = (unsigned)std::min<long long>(G.P, (long long)c->n_cu * 8);
-            hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
-            HIPCHK(c, hipGetLastError());
+            int S = 1;
+            for (int i : G.members) S = std::max(S, pl.jobs[i].s.g.S);
+            if ((rc = enqueue_exact(c, ea, G.P, S))) return rc;
             continue;
         }
         unsigned long long* cg = ctr + 16 + 4 * gi;
@@ -749,9 +786,9 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
         ea.stacks = dtab;
         ea.list = (const SlotRef*)c->fblist.p;
         ea.n_dev = (const int*)(ctr + 4);
-        const unsigned nb = (unsigned)std::min<long long>(pl.P, (long long)c->n_cu * 8);
-        hipLaunchKernelGGL(exact64_kernel, dim3(nb), dim3(256), 0, c->stream, ea);
-        HIPCHK(c, hipGetLastError());
+        int S = 1;
+        for (const StackJob& j : pl.jobs) S = std::max(S, j.s.g.S);
+        if ((rc = enqueue_exact(c, ea, pl.P, S))) return rc;
     }
     if (timed) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     return CLM_OK;
@@ -834,7 +871,7 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
         add(sp, sizeof sp);
     }
     const void* bufs[] = {c->table.p, c->ctr.p, c->ulist.p, c->erecs.p, c->plist.p, c->tiles2.p, c->fblist.p,
-                          c->tw_tiles.p, c->tw_plist.p};
+                          c->tw_tiles.p, c->tw_plist.p, c->ex_v.p, c->ex_k.p};
     add(bufs, sizeof bufs);
     for (const Group& G : pl.groups) {
         const void* t = G.tc ? G.tc->tiles.p : nullptr;
@@ -1143,6 +1180,8 @@ void clm_destroy(clm_ctx* c) {
     for (TileCache& tc : c->tcache) release(tc.tiles);
     release(c->tune_tiles);
     release(c->tw_tiles);
+    release(c->ex_v);
+    release(c->ex_k);
     release(c->tw_plist);
     DevBuf* snb[] = {&c->sn_h1, &c->sn_h2, &c->sn_z, &c->sn_y, &c->sn_dy, &c->sn_dz, &c->sn_dz2, &c->sn_dz1};
     for (DevBuf* b : snb) release(*b);

@@ -48,6 +48,8 @@ struct clm_ctx {
         w_shiftv;
     // refinement of uncertain pairs by the FP32 twins: one-pair tiles and pairs
     clmh::DevBuf tw_tiles, tw_plist;
+    // FP64 exhaustive search: per (item, shift range) best values and keys
+    clmh::DevBuf ex_v, ex_k;
     // shift network: activations and gradients of the last batch
     clmh::DevBuf sn_h1, sn_h2, sn_z, sn_y, sn_dy, sn_dz, sn_dz2, sn_dz1;
     // autotuned variants: shape key -> (ct, W, lag)

@@ -384,14 +384,25 @@ __global__ void build_pass2_kernel(const StackDev* stacks, const int* n_unc, con
 }
 
 // ------------------------------------------------------ FP64 exhaustive
-// One workgroup per pair.  Candidates are enumerated in the reference's tie
-// order q = (j*2L + l2p)*L + l1; the best (value, q) wins.
+// Every candidate of a pair in FP64 with the oracle's arithmetic: for shift
+// j and map m the rows a(r) = P1(r).*ph_j(r) (conj(P1) for C2) are formed
+// exactly as the oracle forms them, and each entry accumulates
+// acc = acc + (a.re*b.re + a.im*b.im) over r = 0..R-1 in order, so every value
+// is bit-identical to the oracle's.  Computed as 64 x 64 output tiles (rows
+// l1, columns l2 of one map) with 16 outputs per thread and the operands of
+// 32 consecutive r staged in LDS.  The best (value, tie key) wins; the key
+// q = (j*2L + l2p)*L + l1 is the reference's first-max order.
 struct Exact64Args {
     const StackDev* stacks;   // the whole table (SlotRef.stack indexes it)
     int stack0, nstacks;      // without a list: stacks [stack0, stack0 + nstacks)
-    const SlotRef* list;      // pairs to compute, or null: every pair of every stack
+    const SlotRef* list;      // pairs to compute, or null: every pair of those stacks
     const int* n_dev;         // list length (device)
-    long long n_all;          // total pairs when list is null
+    long long n_all;          // pairs when list is null
+    long long item0;          // this launch handles items [item0, item0 + item_cap)
+    long long item_cap;
+    int sj;                   // shift ranges per item (gridDim.y); 1: a workgroup does the whole item
+    double* sv;               // sj > 1: best value and key per (item - item0, range)
+    long long* sk;
 };
 
 __device__ inline void slot_to_pair(long long s, int K, int& k1, int& k2) {
@@ -407,48 +418,130 @@ __device__ inline void slot_to_pair(long long s, int K, int& k1, int& k2) {
     k2 = (int)(s - first(a)) + a + 1;
 }
 
+__device__ inline void exact_item(const Exact64Args& a, long long it, int& stk, int& slot) {
+    if (a.list) { stk = a.list[it].stack; slot = a.list[it].slot; }
+    else {
+        stk = a.stack0 + (a.nstacks > 1 ? find_stack(a.stacks + a.stack0, a.nstacks, it) : 0);
+        slot = (int)(it - a.stacks[stk].pair_base);
+    }
+}
+
 __global__ __launch_bounds__(256) void exact64_kernel(Exact64Args a) {
-    __shared__ double s_v[256];
-    __shared__ long long s_k[256];
-    const long long n = a.list ? (long long)*a.n_dev : a.n_all;
-    for (long long it = blockIdx.x; it < n; it += gridDim.x) {
+    __shared__ double2 sA[32][64];  // a(r) of 64 rows l1, 32 consecutive r
+    __shared__ double2 sB[32][64];  // P2(r) of 64 columns l2
+    const long long n_tot = a.list ? (long long)*a.n_dev : a.n_all;
+    const long long n = n_tot < a.item0 + a.item_cap ? n_tot : a.item0 + a.item_cap;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (long long it = a.item0 + blockIdx.x; it < n; it += gridDim.x) {
         int stk, slot;
-        if (a.list) { stk = a.list[it].stack; slot = a.list[it].slot; }
-        else {
-            stk = a.stack0 + (a.nstacks > 1 ? find_stack(a.stacks + a.stack0, a.nstacks, it) : 0);
-            slot = (int)(it - a.stacks[stk].pair_base);
-        }
+        exact_item(a, it, stk, slot);
         const StackDev& sd = a.stacks[stk];
-        const Geom& g = sd.g;
-        const long long NQ = (long long)g.S * 2 * g.L * g.L;
+        const int S = sd.g.S, L = sd.g.L, R = sd.g.R;
         int k1, k2;
-        slot_to_pair(slot, g.K, k1, k2);
+        slot_to_pair(slot, sd.g.K, k1, k2);
+        const int j0 = (int)((long long)blockIdx.y * S / a.sj), j1 = (int)((long long)(blockIdx.y + 1) * S / a.sj);
+        const int NT = (L + 63) / 64;
+        const double2* P1 = sd.P64 + (size_t)k1 * L * R;
+        const double2* P2 = sd.P64 + (size_t)k2 * L * R;
         double bv = -INFINITY;
         long long bk = 0x7fffffffffffffffLL;
-        for (long long qq = threadIdx.x; qq < NQ; qq += blockDim.x) {
-            const int l1 = (int)(qq % g.L);
-            const int l2p = (int)((qq / g.L) % (2 * g.L));
-            const int j = (int)(qq / ((long long)2 * g.L * g.L));
-            const double2* P1 = sd.P64 + ((size_t)k1 * g.L + l1) * g.R;
-            const double2* P2 = sd.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
-            const double v = entry64(P1, P2, sd.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
-            if (!isnan(v) && better64(v, qq, bv, bk)) { bv = v; bk = qq; }
+        for (int j = j0; j < j1; j++) {
+            const double2* ph = sd.ph64 + (size_t)j * R;
+            for (int m = 0; m < 2; m++)
+                for (int rt = 0; rt < NT; rt++)
+                    for (int ct = 0; ct < NT; ct++) {
+                        double acc[4][4];
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
+                        for (int r0 = 0; r0 < R; r0 += 32) {
+                            const int rn = R - r0 < 32 ? R - r0 : 32;
+                            __syncthreads();
+                            for (int idx = tid; idx < 32 * 64; idx += 256) {
+                                const int rr = idx >> 6, col = idx & 63;
+                                double2 av = make_double2(0.0, 0.0), bb = make_double2(0.0, 0.0);
+                                const int l1 = rt * 64 + col, l2 = ct * 64 + col;
+                                if (rr < rn && l1 < L) {
+                                    const double2 pv = P1[(size_t)l1 * R + r0 + rr], qv = ph[r0 + rr];
+                                    const double pr = pv.x, pi = pv.y, qr = qv.x, qi = qv.y;
+                                    if (m == 0) av = make_double2(pr * qr - pi * qi, pr * qi + pi * qr);
+                                    else        av = make_double2(pr * qr + pi * qi, pr * qi - pi * qr);
+                                }
+                                if (rr < rn && l2 < L) bb = P2[(size_t)l2 * R + r0 + rr];
+                                sA[rr][col] = av;
+                                sB[rr][col] = bb;
+                            }
+                            __syncthreads();
+                            for (int rr = 0; rr < rn; rr++) {
+                                double2 A[4], B[4];
+#pragma unroll
+                                for (int i = 0; i < 4; i++) {
+                                    A[i] = sA[rr][ty + 16 * i];
+                                    B[i] = sB[rr][tx + 16 * i];
+                                }
+#pragma unroll
+                                for (int i = 0; i < 4; i++)
+#pragma unroll
+                                    for (int k = 0; k < 4; k++) {
+                                        const double t = A[i].x * B[k].x, u = A[i].y * B[k].y;
+                                        acc[i][k] = acc[i][k] + (t + u);
+                                    }
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int k = 0; k < 4; k++) {
+                                const int l1 = rt * 64 + ty + 16 * i, l2 = ct * 64 + tx + 16 * k;
+                                if (l1 < L && l2 < L) {
+                                    const double v = 2.0 * acc[i][k];
+                                    const long long key = tie_key(j, l2 + m * L, l1, L);
+                                    if (!isnan(v) && better64(v, key, bv, bk)) { bv = v; bk = key; }
+                                }
+                            }
+                    }
         }
-        s_v[threadIdx.x] = bv;
-        s_k[threadIdx.x] = bk;
+        // workgroup best (LDS of the tiles reused)
         __syncthreads();
-        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) {
-                if (better64(s_v[threadIdx.x + o], s_k[threadIdx.x + o], s_v[threadIdx.x], s_k[threadIdx.x])) {
-                    s_v[threadIdx.x] = s_v[threadIdx.x + o];
-                    s_k[threadIdx.x] = s_k[threadIdx.x + o];
-                }
+        double* s_v = (double*)&sA[0][0];
+        long long* s_k = (long long*)&sB[0][0];
+        s_v[tid] = bv;
+        s_k[tid] = bk;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o && better64(s_v[tid + o], s_k[tid + o], s_v[tid], s_k[tid])) {
+                s_v[tid] = s_v[tid + o];
+                s_k[tid] = s_k[tid + o];
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) write_result(sd, slot, s_v[0], s_k[0]);
+        if (tid == 0) {
+            if (a.sj == 1) write_result(sd, slot, s_v[0], s_k[0]);
+            else {
+                const long long e = (it - a.item0) * a.sj + blockIdx.y;
+                a.sv[e] = s_v[0];
+                a.sk[e] = s_k[0];
+            }
+        }
         __syncthreads();
     }
+}
+
+// Best over an item's shift ranges (sj > 1), one thread per item.
+__global__ void exact64_reduce_kernel(Exact64Args a) {
+    const long long n_tot = a.list ? (long long)*a.n_dev : a.n_all;
+    const long long it = a.item0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n_tot || it >= a.item0 + a.item_cap) return;
+    double bv = -INFINITY;
+    long long bk = 0x7fffffffffffffffLL;
+    for (int y = 0; y < a.sj; y++) {
+        const long long e = (it - a.item0) * a.sj + y;
+        if (better64(a.sv[e], a.sk[e], bv, bk)) { bv = a.sv[e]; bk = a.sk[e]; }
+    }
+    int stk, slot;
+    exact_item(a, it, stk, slot);
+    write_result(a.stacks[stk], slot, bv, bk);
 }
 
 // FP32 operand planes [K][2][Lp][Rp] of a stack from its FP32 half lines

@@ -50,10 +50,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic variant)")
     ap.add_argument("--tune-pairs", type=int, default=200000)
     ap.add_argument("--flags", type=int, default=0, help="clm_params.reserved bits (1: no lagged schedule)")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped", "configs"],
                     help="pairs: one BF16 call on the workload and nothing else (for counter "
                          "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration); "
-                         "grouped: the 64-stack uneven batch in one graph-captured grouped job")
+                         "grouped: the 64-stack uneven batch in one graph-captured grouped job; "
+                         "configs: the K=100 and K=1000 workloads in every precision against the oracle")
     ap.add_argument("--stacks", type=int, default=64, help="grouped mode: number of stacks")
     ap.add_argument("--grouped-sample", type=int, default=40, help="grouped mode: oracle pairs per stack")
     args = ap.parse_args()
@@ -66,6 +67,8 @@ def main():
 
     if args.mode == "grouped":
         return grouped_bench(args)
+    if args.mode == "configs":
+        return configs_bench(args)
 
     if args.mode == "copy":
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -240,6 +243,67 @@ def main():
         "stats": all_stats[-1],
     }
     print(json.dumps(out))
+
+
+def configs_bench(args):
+    """K=100 (65 px, n_r=33, no shifts, SNR 1, max_shift 0) with every pair
+    checked against the FP64 oracle, and K=1000 (89 px, integer shifts in
+    [-4,4], SNR 1/8, n_r=45, max_shift 6, step 1: 13 shifts) with a seeded
+    sample of 2000 pairs, in FP32, BF16, FP16 and FP64."""
+    import torch
+
+    from aspire_amd import sim
+    from aspire_amd.clmatrix import clmatrix_records
+    from oracle import clm_oracle
+
+    cases = [dict(name="K=100 n=65 n_r=33 T=360 SNR=1 no shifts, max_shift=0", K=100, n=65, n_r=33, snr=1.0,
+                  shifts=0, kind="int", ms=0, step=1, seed=1001, sample=None),
+             dict(name="K=1000 n=89 n_r=45 T=360 SNR=1/8 int shifts [-4,4], max_shift=6 step=1 (13 shifts)",
+                  K=1000, n=89, n_r=45, snr=1 / 8, shifts=4, kind="int", ms=6, step=1, seed=1002, sample=2000)]
+    out = []
+    for cs in cases:
+        st = sim.make_stack(K=cs["K"], n=cs["n"], n_r=cs["n_r"], n_theta=360, snr=cs["snr"],
+                            max_shift_2d=cs["shifts"], shift_kind=cs["kind"], seed=cs["seed"], device="cuda")
+        pf = st.pf
+        K = cs["K"]
+        P = K * (K - 1) // 2
+        k1, k2 = np.triu_indices(K, 1)
+        sel = np.arange(P) if cs["sample"] is None else np.sort(
+            np.random.default_rng(42).choice(P, size=cs["sample"], replace=False))
+        host = pf.cpu().numpy()
+        Pr, _ = clm_oracle.preprocess(host)
+        nth = clm_oracle.lib().clmo_max_threads()
+        t = time.perf_counter()
+        ora = clm_oracle.pairs(Pr, cs["ms"], cs["step"], k1[sel].astype(np.int32), k2[sel].astype(np.int32),
+                               nthreads=nth)
+        cpu_s = time.perf_counter() - t
+        row = dict(workload=cs["name"], pairs=P, checked_pairs=int(sel.size),
+                   cpu_oracle_pairs_per_s=sel.size / cpu_s, cpu_threads=int(nth), precisions={})
+        for prec in ("fp32", "bf16", "fp16", "fp64"):
+            run = lambda: clmatrix_records(pf, cs["ms"], cs["step"], precision=prec)
+            run()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            rec, stats = run()
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t
+            g = {k: v.cpu().numpy()[sel] for k, v in rec.items()}
+            same = (g["cl1"] == ora["cl1"]) & (g["cl2"] == ora["cl2"]) & (g["j"] == ora["j"])
+            near = 0
+            for i in np.flatnonzero(~same):
+                v = clm_oracle.entry(Pr, cs["ms"], cs["step"], k1[sel][i], k2[sel][i], g["j"][i], g["cl1"][i] - 1,
+                                     g["cl2"][i] - 1)
+                if ora["best"][i] - v <= 1e-5 * abs(ora["best"][i]):
+                    near += 1
+            m = same & (ora["best"] != 0)
+            crel = float(np.max(np.abs(g["corr"][m] - ora["best"][m]) / np.abs(ora["best"][m]))) if m.any() else 0.0
+            row["precisions"][prec] = dict(seconds=sec, pairs_per_s=P / sec, clstack_match_pct=100 * float(same.mean()),
+                                           near_tie_disagreements=near, corr_rel_max=crel,
+                                           corr_bit_identical=bool(np.array_equal(g["corr"][same], ora["best"][same])),
+                                           uncertain=stats["uncertain_pairs"], refined=stats["refined_pairs"])
+            log(f"{cs['name']} {prec}: {row['precisions'][prec]}")
+        out.append(row)
+    print(json.dumps({"mode": "configs", "cases": out}))
 
 
 def grouped_bench(args):
