@@ -637,6 +637,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
         pa.erecs = nullptr;
         pa.CPP = G.CPP;
         pa.lag = G.lag;
+        pa.prio = (pl.jobs[G.members[0]].p.reserved & 4) ? 0 : 1;
         long long max_tiles;
         if (G.sel) {
             const StackJob& j = pl.jobs[G.members[0]];
@@ -767,6 +768,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
         pa.erecs = (EmitRec*)c->erecs.p;
         pa.CPP = G.CPP;
         pa.lag = G.lag;
+        pa.prio = (pl.jobs[G.members[0]].p.reserved & 4) ? 0 : 1;
         const unsigned g2 = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, cap);
         hipLaunchKernelGGL(G.fn2, dim3(g2), dim3(64 * G.W), G.lds, c->stream, pa);
         HIPCHK(c, hipGetLastError());
@@ -1090,6 +1092,7 @@ int autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int sample_pairs
             pa.counter = (int*)((unsigned long long*)c->ctr.p + 16);
             pa.CPP = G.CPP;
             pa.lag = G.lag;
+            pa.prio = (p->reserved & 4) ? 0 : 1;
             const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, (long long)h.size());
             float ms = 0.f;
             for (int rep = 0; rep < 2; rep++) {  // first launch warms caches and code

@@ -105,6 +105,8 @@ struct PairArgs {
     EmitRec* erecs;        // indexed by u (second pass)
     int CPP;               // waves per pair (column chunks); same for all stacks
     int lag;               // 1: one wave per SIMD runs one A-tile behind
+    int prio;              // 1: raised wave priority while issuing MFMAs (the other
+                           // wave on the SIMD runs its epilogue meanwhile)
 };
 
 constexpr int WCAP = 64;   // per-wave list of candidate cells (one per lane)
@@ -330,6 +332,7 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
                 for (int ct = 0; ct < CT; ct++) epi(ct, at - 1);
                 keep_cells(at - 1);
             }
+            if (a.prio) __builtin_amdgcn_s_setprio(1);
             if (active) {
                 const char* arow = (const char*)cur + lc * CPR * 16;
 #pragma unroll
@@ -356,6 +359,7 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
                     }
                 }
             }
+            if (a.prio) __builtin_amdgcn_s_setprio(0);
             if (!lag && active) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++) epi(ct, at);

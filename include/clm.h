@@ -70,7 +70,8 @@ typedef struct clm_params {
     int32_t reserved;    /* tuning bits: bit 0 disables the lagged wave schedule;
                             bit 1 disables the FP32 refinement of uncertain
                             BF16/FP16 pairs (they take a second BF16/FP16
-                            pass instead)                                   */
+                            pass instead); bit 2 disables the raised wave
+                            priority during MFMA issue                      */
 } clm_params;
 
 typedef struct clm_stats {
