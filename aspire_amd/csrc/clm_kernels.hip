@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         const StackDev& sd = a.stacks[stk];
         const Geom& g = sd.g;
         const int kq = g.Rp / 4;  // multiple of 4
-        const PairRec rec = sd.recs[slot];
+        const PairRec& rec = sd.recs[slot];
         const int k1 = rec.k1, k2 = rec.k2;
         const EmitRec* er = emit ? a.erecs + item : nullptr;
         const int cmode = emit ? 1 : 0;

@@ -202,9 +202,12 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         // loads its 16-byte chunks of tile at+1 into registers one step
         // ahead and writes them to LDS at the start of the next step.
         const int4* asrc = (const int4*)((const char*)sd.astk + (size_t)k1 * g.NAT * AT::BYTES);
-        int4 pre[NPT];
+        // native vectors: copies of HIP's int4 wrapper kept this buffer in
+        // private memory
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        v4i pre[NPT];
         auto load_pre = [&](int at) {
-            const int4* src = asrc + (size_t)at * NCH;
+            const v4i* src = (const v4i*)(asrc + (size_t)at * NCH);
 #pragma unroll
             for (int j = 0; j < NPT; j++) {
                 const int q = tid + j * (int)blockDim.x;
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
 #pragma unroll
             for (int j = 0; j < NPT; j++) {
                 const int q = tid + j * (int)blockDim.x;
-                if (q < NCH) buf[q] = pre[j];
+                if (q < NCH) ((v4i*)buf)[q] = pre[j];
             }
         };
         load_pre(0);
