@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libclm.so")
 
 CLM_FP32, CLM_BF16_RESCORE, CLM_FP16_RESCORE, CLM_FP64_REF = 0, 1, 2, 3
 CLM_C128, CLM_C64 = 0, 1
+CLM_R64, CLM_R32 = 0, 1
 
 PRECISIONS = {
     "fp32": CLM_FP32,
@@ -28,6 +29,7 @@ EXPORTS = (
     "clm_clmatrix_grouped", "clm_autotune", "clm_clmatrix_windowed",
     "clm_shiftnet_nparam", "clm_shiftnet_features", "clm_shiftnet_forward",
     "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
+    "clm_pft_device", "clm_pft", "clm_pft_freqs",
 )
 
 CLM_GROUP_GRAPH = 1
@@ -108,6 +110,29 @@ class ClmStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ClmPftParams(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("K", ctypes.c_int32),
+        ("n_r", ctypes.c_int32),
+        ("n_theta", ctypes.c_int32),
+        ("img_dtype", ctypes.c_int32),
+        ("pf_dtype", ctypes.c_int32),
+    ]
+
+
+class ClmPftStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_tables", ctypes.c_float),
+        ("ms_stage", ctypes.c_float),
+        ("ms_pft", ctypes.c_float),
+        ("chunks", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -168,6 +193,12 @@ def load():
     L.clm_shift_equations_csc.restype = ctypes.c_int
     L.clm_shift_equations_csc.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp, vp, vp,
                                           ctypes.POINTER(ctypes.c_int64)]
+    L.clm_pft_device.restype = ctypes.c_int
+    L.clm_pft_device.argtypes = [vp, ctypes.POINTER(ClmPftParams), vp, vp, ctypes.POINTER(ClmPftStats)]
+    L.clm_pft.restype = ctypes.c_int
+    L.clm_pft.argtypes = [vp, ctypes.POINTER(ClmPftParams), vp, vp, ctypes.POINTER(ClmPftStats)]
+    L.clm_pft_freqs.restype = ctypes.c_int
+    L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L
     return L
 

@@ -1,0 +1,326 @@
+// pft.hip -- polar Fourier transform of a projection stack on the device:
+// the step before the common-lines search.
+//
+// Replaces cryo_pft(p, n_r, n_theta) (fourier/nufft/cryo_pft.m:1-75) with its
+// exact evaluation, the non-uniform DFT of fourier/nufft/nudft2.m:40-47 (the
+// reference's own fallback when no NUFFT library is installed):
+//
+//   freqs(m) = ((k)*w0*sin(j*dtheta), (k)*w0*cos(j*dtheta)),  m = j*n_r + k,
+//       w0 = 2*pi/(2*n_r - 1), dtheta = 2*pi/n_theta, j < n_theta/2  (:49-58)
+//   pf(k, j, i) = sum_{u,v} p(u, v, i) * exp(+1i*(fx(m)*g(u) + fy(m)*g(v)))
+//       g(t) = ceil(t - n/2)      (nudft2.m:40-47 evaluated at -freqs, :70)
+//   pf(:, n_theta/2 + j, :) = conj(pf(:, j, :))                        (:75)
+//
+// u is the first (row) index of an image, paired with fx = r*sin(theta); v is
+// the second, paired with fy = r*cos(theta).
+//
+// On the device this is a contraction over the n*n pixels of a complex weight
+// matrix E (M x n^2, M = n_r*n_theta/2) with the real image stack (n^2 x K):
+// FP64 MFMA (v_mfma_f64_16x16x4f64).  E is never stored: the exponential
+// factors per axis, E(m,(u,v)) = ax(u,m) * ay(v,m), come from two small tables
+// (n x M complex each) and each lane forms its operand with one complex
+// product.  Images are first staged pixel-major (pixel rows of Kp images,
+// FP64) so the B fragments load as 128-byte rows.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "clm_ctx.h"
+
+namespace pft {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int MB = 128;  // polar points per workgroup (4 waves x 32)
+constexpr int KB = 64;   // images per workgroup (4 column tiles of 16)
+constexpr int KCHUNK = 8192;  // images per staged chunk
+
+struct Geo {
+    int n, n_r, n_theta, M, Mp;  // M = n_r * n_theta/2 polar points, Mp = roundup(M, MB)
+    int nup;                     // roundup(n, 4): pixel rows per image column in the staged stack
+    int Kc, Kp;                  // images in this chunk, roundup(Kc, KB)
+};
+
+// ax[u*Mp + m] = exp(+1i*fx(m)*g(u)) (u < nup; zero for u >= n or m >= M),
+// ay[v*Mp + m] = exp(+1i*fy(m)*g(v)) (v < n).  grid: (Mp/256, nup + n).
+__global__ void tables_kernel(Geo g, double2* ax, double2* ay) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (m >= g.Mp) return;
+    const bool is_x = t < g.nup;
+    const int idx = is_x ? t : t - g.nup;
+    double2 out = make_double2(0.0, 0.0);
+    if (m < g.M && idx < g.n) {
+        const int k = m % g.n_r, j = m / g.n_r;
+        const double omega0 = 2.0 * M_PI / (2.0 * g.n_r - 1.0);  // cryo_pft.m:49
+        const double dtheta = 2.0 * M_PI / g.n_theta;             // cryo_pft.m:50
+        const double f = is_x ? (k * omega0) * sin(j * dtheta) : (k * omega0) * cos(j * dtheta);
+        const double gt = ceil(-0.5 * g.n + idx);                 // nudft2.m:40
+        double s, c;
+        sincos(f * gt, &s, &c);
+        out = make_double2(c, s);
+    }
+    (is_x ? ax : ay)[(size_t)idx * g.Mp + m] = out;
+}
+
+// Staged stack: xt[(v*nup + u)*Kp + i] = p(u, v, k0 + i), zero-padded in u and
+// i.  One workgroup moves 64 pixel rows x 64 images through LDS so that both
+// the reads (along u) and the writes (along i) are contiguous.
+template <typename T>
+__global__ __launch_bounds__(256) void stage_kernel(Geo g, const T* __restrict__ img, long long k0, int K,
+                                                    double* __restrict__ xt) {
+    __shared__ double tile[64][65];
+    const int q0 = blockIdx.x * 64;  // staged pixel row
+    const int i0 = blockIdx.y * 64;  // image within the chunk
+    const int npix = g.n * g.nup;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int i = i0 + r, q = q0 + tx;
+        double val = 0.0;
+        if (q < npix && i < g.Kc && k0 + i < K) {
+            const int v = q / g.nup, u = q - v * g.nup;
+            if (u < g.n) val = (double)img[(size_t)(k0 + i) * g.n * g.n + (size_t)v * g.n + u];
+        }
+        tile[r][tx] = val;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int q = q0 + r, i = i0 + tx;
+        if (q < npix && i < g.Kp) xt[(size_t)q * g.Kp + i] = tile[tx][r];
+    }
+}
+
+template <typename OutT>
+__device__ inline void put(OutT* pf, size_t at, double re, double im);
+template <>
+__device__ inline void put<double2>(double2* pf, size_t at, double re, double im) {
+    pf[at] = make_double2(re, im);
+}
+template <>
+__device__ inline void put<float2>(float2* pf, size_t at, double re, double im) {
+    pf[at] = make_float2((float)re, (float)im);
+}
+
+// One workgroup: MB polar points x KB images; wave w takes points
+// [32w, 32w + 32) of the block as two 16-row tiles, each split into its real
+// and imaginary weight rows, against 4 column tiles of 16 images: 16
+// accumulator tiles (16 x 16 FP64 each).  Each step contracts 4 pixels (one
+// MFMA k-block): lane l holds pixel u0 + l/16 for point (l % 16) of a row
+// tile (A) and image (l % 16) of a column tile (B).
+template <typename OutT>
+__global__ __launch_bounds__(256, 2) void pft_kernel(Geo g, const double* __restrict__ xt,
+                                                  const double2* __restrict__ ax, const double2* __restrict__ ay,
+                                                  OutT* __restrict__ pf, long long k0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int mA = blockIdx.x * MB + wave * 32 + r16, mB = mA + 16;
+    const int ib = blockIdx.y * KB;
+    d4 acc[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][b][c] = d4{0.0, 0.0, 0.0, 0.0};
+    // Steps s = (v, u0): pixels (u0 .. u0+3, v).  The operands of step s+1
+    // are loaded while step s multiplies; the last step reloads its own
+    // (clamped) operands.
+    const int nu = g.nup / 4;
+    const int nsteps = g.n * nu;
+    auto load = [&](int v, int u0, double2& xA, double2& xB, double2& yA, double2& yB, double (&bx)[4]) {
+        const int u = u0 + kq;
+        xA = ax[(size_t)u * g.Mp + mA];
+        xB = ax[(size_t)u * g.Mp + mB];
+        yA = ay[(size_t)v * g.Mp + mA];
+        yB = ay[(size_t)v * g.Mp + mB];
+        const double* xr = xt + ((size_t)v * g.nup + u) * g.Kp + ib + r16;
+#pragma unroll
+        for (int c = 0; c < 4; c++) bx[c] = xr[16 * c];
+    };
+    double2 xA, xB, yA, yB;
+    double bx[4];
+    load(0, 0, xA, xB, yA, yB, bx);
+    int v = 0, u0 = 0;
+    for (int s = 0; s < nsteps; s++) {
+        int vn = v, un = u0 + 4;
+        if (un == g.nup) { un = 0; vn++; }
+        if (s + 1 == nsteps) { vn = v; un = u0; }
+        double2 nxA, nxB, nyA, nyB;
+        double nbx[4];
+        load(vn, un, nxA, nxB, nyA, nyB, nbx);
+        // E = ax * ay (complex)
+        const double eAr = xA.x * yA.x - xA.y * yA.y, eAi = xA.x * yA.y + xA.y * yA.x;
+        const double eBr = xB.x * yB.x - xB.y * yB.y, eBi = xB.x * yB.y + xB.y * yB.x;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            acc[0][0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eAr, bx[c], acc[0][0][c], 0, 0, 0);
+            acc[0][1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eAi, bx[c], acc[0][1][c], 0, 0, 0);
+            acc[1][0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eBr, bx[c], acc[1][0][c], 0, 0, 0);
+            acc[1][1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eBi, bx[c], acc[1][1][c], 0, 0, 0);
+        }
+        xA = nxA;
+        xB = nxB;
+        yA = nyA;
+        yB = nyB;
+#pragma unroll
+        for (int c = 0; c < 4; c++) bx[c] = nbx[c];
+        v = vn;
+        u0 = un;
+    }
+    // D tile layout of the FP64 16x16x4 MFMA: lane l, element t holds row
+    // 4*t + l/16, column l % 16 (one row per element, unlike the FP32 tiles).
+    const int L = g.n_theta / 2;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int m = blockIdx.x * MB + wave * 32 + a * 16 + 4 * t + kq;
+            if (m >= g.M) continue;
+            const int k = m % g.n_r, j = m / g.n_r;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int i = ib + 16 * c + r16;
+                if (i >= g.Kc) continue;
+                const size_t img = (size_t)(k0 + i) * g.n_theta;
+                const double re = acc[a][0][c][t], im = acc[a][1][c][t];
+                put<OutT>(pf, (img + j) * g.n_r + k, re, im);
+                put<OutT>(pf, (img + j + L) * g.n_r + k, re, -im);
+            }
+        }
+}
+
+}  // namespace pft
+
+using namespace clmh;
+
+namespace {
+
+int pft_check(clm_ctx* c, const clm_pft_params* p) {
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    if (p->n_theta <= 0 || p->n_theta % 2) return fail(c, CLM_E_ARG, "`n_theta` must be even.");
+    if (p->n < 1 || p->n > 4096) return fail(c, CLM_E_UNSUP, "image side n = %d out of range [1, 4096]", p->n);
+    if (p->n_r < 1 || p->K < 0) return fail(c, CLM_E_ARG, "invalid n_r = %d or K = %d", p->n_r, p->K);
+    if ((long long)p->n_r * (p->n_theta / 2) > (1LL << 26))
+        return fail(c, CLM_E_UNSUP, "n_r * n_theta/2 = %lld polar points exceeds 2^26",
+                    (long long)p->n_r * (p->n_theta / 2));
+    if (p->img_dtype != CLM_R64 && p->img_dtype != CLM_R32)
+        return fail(c, CLM_E_ARG, "unknown image dtype %d", p->img_dtype);
+    if (p->pf_dtype != CLM_C128 && p->pf_dtype != CLM_C64) return fail(c, CLM_E_ARG, "unknown pf dtype %d", p->pf_dtype);
+    return CLM_OK;
+}
+
+int pft_run(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, clm_pft_stats* st) {
+    using namespace pft;
+    Geo g{};
+    g.n = p->n;
+    g.n_r = p->n_r;
+    g.n_theta = p->n_theta;
+    g.M = p->n_r * (p->n_theta / 2);
+    g.Mp = (g.M + MB - 1) / MB * MB;
+    g.nup = (p->n + 3) / 4 * 4;
+    const int K = p->K;
+    const int kc = K < KCHUNK ? K : KCHUNK;
+    const int kcp = (kc + KB - 1) / KB * KB;
+    int rc;
+    if ((rc = ensure(c, c->pft_ax, (size_t)g.nup * g.Mp * sizeof(double2))) ||
+        (rc = ensure(c, c->pft_ay, (size_t)g.n * g.Mp * sizeof(double2))) ||
+        (rc = ensure(c, c->pft_xt, (size_t)g.n * g.nup * (size_t)kcp * sizeof(double))))
+        return rc;
+    hipEvent_t* ev = c->ev;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    tables_kernel<<<dim3(g.Mp / 256 + (g.Mp % 256 != 0), g.nup + g.n), 256, 0, c->stream>>>(
+        g, (double2*)c->pft_ax.p, (double2*)c->pft_ay.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    float ms_stage = 0.f, ms_pft = 0.f;
+    int chunks = 0;
+    for (long long k0 = 0; k0 < K; k0 += KCHUNK) {
+        g.Kc = (int)((K - k0) < KCHUNK ? (K - k0) : KCHUNK);
+        g.Kp = (g.Kc + KB - 1) / KB * KB;
+        const int npix = g.n * g.nup;
+        dim3 sgrid((npix + 63) / 64, g.Kp / 64);
+        HIPCHK(c, hipEventRecord(ev[2], c->stream));
+        if (p->img_dtype == CLM_R64)
+            stage_kernel<double><<<sgrid, 256, 0, c->stream>>>(g, (const double*)d_img, k0, K, (double*)c->pft_xt.p);
+        else
+            stage_kernel<float><<<sgrid, 256, 0, c->stream>>>(g, (const float*)d_img, k0, K, (double*)c->pft_xt.p);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[3], c->stream));
+        dim3 grid(g.Mp / MB, g.Kp / KB);
+        if (p->pf_dtype == CLM_C128)
+            pft_kernel<double2><<<grid, 256, 0, c->stream>>>(g, (const double*)c->pft_xt.p, (const double2*)c->pft_ax.p,
+                                                               (const double2*)c->pft_ay.p, (double2*)d_pf, k0);
+        else
+            pft_kernel<float2><<<grid, 256, 0, c->stream>>>(g, (const double*)c->pft_xt.p, (const double2*)c->pft_ax.p,
+                                                              (const double2*)c->pft_ay.p, (float2*)d_pf, k0);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[4], c->stream));
+        HIPCHK(c, hipEventSynchronize(ev[4]));
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, ev[2], ev[3]));
+        HIPCHK(c, hipEventElapsedTime(&b, ev[3], ev[4]));
+        ms_stage += a;
+        ms_pft += b;
+        chunks++;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[1]));
+        st->ms_tables = t;
+        st->ms_stage = ms_stage;
+        st->ms_pft = ms_pft;
+        st->chunks = chunks;
+    }
+    return CLM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clm_pft_freqs(int32_t n_r, int32_t n_theta, double* freqs) {
+    if (n_r < 1 || n_theta <= 0 || n_theta % 2 || !freqs) return CLM_E_ARG;
+    const int L = n_theta / 2;
+    const size_t M = (size_t)n_r * L;
+    const double omega0 = 2.0 * M_PI / (2.0 * n_r - 1.0);
+    const double dtheta = 2.0 * M_PI / n_theta;
+    for (int j = 0; j < L; j++)
+        for (int k = 0; k < n_r; k++) {
+            const size_t m = (size_t)j * n_r + k;
+            freqs[m] = (k * omega0) * sin(j * dtheta);
+            freqs[M + m] = (k * omega0) * cos(j * dtheta);
+        }
+    return CLM_OK;
+}
+
+int clm_pft_device(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, clm_pft_stats* st) {
+    if (!c) return CLM_E_ARG;
+    int rc = pft_check(c, p);
+    if (rc) return rc;
+    if (st) *st = clm_pft_stats{};
+    if (p->K == 0) return CLM_OK;
+    if (!d_img || !d_pf) return fail(c, CLM_E_ARG, "null image or pf pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    return pft_run(c, p, d_img, d_pf, st);
+}
+
+int clm_pft(clm_ctx* c, const clm_pft_params* p, const void* img, void* pf, clm_pft_stats* st) {
+    if (!c) return CLM_E_ARG;
+    int rc = pft_check(c, p);
+    if (rc) return rc;
+    if (st) *st = clm_pft_stats{};
+    if (p->K == 0) return CLM_OK;
+    if (!img || !pf) return fail(c, CLM_E_ARG, "null image or pf pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nimg = (size_t)p->n * p->n * p->K * (p->img_dtype == CLM_R64 ? 8 : 4);
+    const size_t npf = (size_t)p->n_r * p->n_theta * p->K * (p->pf_dtype == CLM_C128 ? 16 : 8);
+    if ((rc = ensure(c, c->pft_img, nimg)) || (rc = ensure(c, c->pft_out, npf))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pft_img.p, img, nimg, hipMemcpyHostToDevice, c->stream));
+    if ((rc = pft_run(c, p, c->pft_img.p, c->pft_out.p, st))) return rc;
+    HIPCHK(c, hipMemcpyAsync(pf, c->pft_out.p, npf, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+}  // extern "C"

@@ -245,13 +245,10 @@ class Stack:
     snr: float | None
 
 
-def make_stack(K: int, n: int, n_r: int, n_theta: int = 360, snr: float | None = None,
-               max_shift_2d: float = 0.0, shift_kind: str = "int", seed: int = 0,
-               n_blobs: int = 20, device=None, chunk_images: int = 1024,
-               dtype: str = "complex128", recenter: bool = False) -> Stack:
-    """A seeded stack: Haar rotations, Gaussian-blob projections, 2-D shifts
-    (integer or uniform in [-max_shift_2d, max_shift_2d]), white noise at
-    ``snr`` (None = clean) and the polar Fourier transform."""
+def make_images(K: int, n: int, snr: float | None = None, max_shift_2d: float = 0.0, shift_kind: str = "int",
+                seed: int = 0, n_blobs: int = 20, device=None, recenter: bool = False):
+    """The images of :func:`make_stack` (K x n x n, imgs[k, u, v] = p(u, v, k)),
+    with their rotations, shifts and phantom: (imgs, rots, shifts, blobs)."""
     rng = np.random.default_rng(seed)
     rots = rand_rots(rng, K)
     blobs = gaussian_blobs(rng, n_blobs, recenter=recenter)
@@ -267,6 +264,17 @@ def make_stack(K: int, n: int, n_r: int, n_theta: int = 360, snr: float | None =
         imgs = add_shifts(imgs, shifts)
     if snr is not None:
         imgs = add_noise(rng, imgs, snr)
+    return imgs, rots, shifts, blobs
+
+
+def make_stack(K: int, n: int, n_r: int, n_theta: int = 360, snr: float | None = None,
+               max_shift_2d: float = 0.0, shift_kind: str = "int", seed: int = 0,
+               n_blobs: int = 20, device=None, chunk_images: int = 1024,
+               dtype: str = "complex128", recenter: bool = False) -> Stack:
+    """A seeded stack: Haar rotations, Gaussian-blob projections, 2-D shifts
+    (integer or uniform in [-max_shift_2d, max_shift_2d]), white noise at
+    ``snr`` (None = clean) and the polar Fourier transform."""
+    imgs, rots, shifts, blobs = make_images(K, n, snr, max_shift_2d, shift_kind, seed, n_blobs, device, recenter)
     if device is None:
         pf = cryo_pft(imgs, n_r, n_theta)
         if dtype == "complex64":

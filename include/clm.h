@@ -274,6 +274,53 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t *cl1,
                             const int32_t *cl2, const double *shift1d,
                             int64_t *jc, int64_t *ir, double *pr, int64_t *nnz);
 
+/*
+ * Polar Fourier transform of a projection stack on the device: the step
+ * before cryo_clmatrix.  Replaces [pf, freqs] = cryo_pft(p, n_r, n_theta)
+ * (fourier/nufft/cryo_pft.m:1-75) with the exact non-uniform DFT that
+ * cryo_pft's nufft2 falls back to (fourier/nufft/nudft2.m:40-47):
+ *   freqs(m,:) = [k*w0*sin(j*dtheta), k*w0*cos(j*dtheta)], m = j*n_r + k,
+ *       w0 = 2*pi/(2*n_r-1), dtheta = 2*pi/n_theta, j < n_theta/2 (:49-58)
+ *   pf(k,j,i) = sum_{u,v} p(u,v,i) exp(+1i*(freqs(m,1)*g(u) + freqs(m,2)*g(v))),
+ *       g(t) = ceil(t - n/2)  (nudft2 evaluated at -freqs, :70)
+ *   pf(:, n_theta/2 + j, :) = conj(pf(:, j, :))  (:75)
+ * Images are real, square n x n, column-major n x n x K (MATLAB layout,
+ * element (u,v,i) at u + n*v + n*n*i), double or single; pf is written in
+ * the layout clm_clmatrix reads (see the top of this file).  Arithmetic is
+ * FP64 throughout (the pf of single images is rounded to complex single at
+ * the end, as MATLAB returns single for single input).
+ */
+enum clm_real_dtype {
+    CLM_R64 = 0, /* double images (MATLAB double) */
+    CLM_R32 = 1  /* single images (MATLAB single) */
+};
+
+typedef struct clm_pft_params {
+    int32_t n;         /* image side (square images, nudft2.m:27-31)       */
+    int32_t K;         /* number of images                                 */
+    int32_t n_r;       /* radial samples per ray                           */
+    int32_t n_theta;   /* rays; must be even (cryo_pft.m:44-46)            */
+    int32_t img_dtype; /* enum clm_real_dtype                              */
+    int32_t pf_dtype;  /* enum clm_dtype of the output                     */
+} clm_pft_params;
+
+typedef struct clm_pft_stats {
+    float ms_tables; /* per-axis exponential tables                 */
+    float ms_stage;  /* images to the pixel-major FP64 staging area */
+    float ms_pft;    /* the contraction (FP64 MFMA)                 */
+    int32_t chunks;  /* image chunks of at most 8192                */
+} clm_pft_stats;
+
+/* Device-buffer form: d_img and d_pf are device pointers on the context's
+ * device; returns once pf is written.  stats may be NULL. */
+int clm_pft_device(clm_ctx *ctx, const clm_pft_params *p, const void *d_img,
+                   void *d_pf, clm_pft_stats *stats);
+/* Host-buffer form (img and pf in host memory). */
+int clm_pft(clm_ctx *ctx, const clm_pft_params *p, const void *img, void *pf,
+            clm_pft_stats *stats);
+/* freqs of cryo_pft.m:52-58, (n_r*n_theta/2) x 2 column-major doubles. */
+int clm_pft_freqs(int32_t n_r, int32_t n_theta, double *freqs);
+
 #ifdef __cplusplus
 }
 #endif

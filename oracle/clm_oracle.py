@@ -54,6 +54,11 @@ def lib():
         L.clmo_equations.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
                                      ctypes.c_int64, _f64p, _i32p]
         L.clmo_max_threads.restype = ctypes.c_int
+        L.clmo_pft_freqs.restype = None
+        L.clmo_pft_freqs.argtypes = [ctypes.c_int, ctypes.c_int, _f64p]
+        L.clmo_pft.restype = None
+        L.clmo_pft.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p,
+                               ctypes.c_int]
         _lib = L
     return _lib
 
@@ -157,3 +162,30 @@ def cryo_clmatrix_cpu(pf, NK=None, verbose=0, max_shift=15, shift_step=1.0, nthr
     emap = np.zeros((K, K))
     emap[k1, k2] = np.arange(1, npairs + 1)
     return clstack, corrstack, eqs, emap, np.zeros((K, K))
+
+
+# ------------------------------------------------------ polar Fourier transform
+def pft_freqs(n_r: int, n_theta: int) -> np.ndarray:
+    """freqs of cryo_pft.m:52-58: (n_r*n_theta/2) x 2."""
+    M = n_r * (n_theta // 2)
+    out = np.zeros(2 * M)
+    lib().clmo_pft_freqs(n_r, n_theta, out)
+    return out.reshape(2, M).T.copy()
+
+
+def cryo_pft(p: np.ndarray, n_r: int, n_theta: int, nthreads: int = 0) -> np.ndarray:
+    """pf (n_r x n_theta x K complex128) of real images p (n x n x K, or n x n)
+    by the exact NUDFT of cryo_pft.m:49-75 / nudft2.m:37-47 (pft_oracle.c)."""
+    p = np.asarray(p, dtype=np.float64)
+    if p.ndim == 2:
+        p = p[:, :, None]
+    n, n2, K = p.shape
+    if n != n2:
+        raise ValueError("Only square images supported.")
+    if n_theta % 2:
+        raise ValueError("`n_theta` must be even.")
+    flat = np.ascontiguousarray(p.transpose(2, 1, 0)).ravel()  # column-major n x n x K
+    out = np.zeros(2 * n_r * n_theta * K)
+    lib().clmo_pft(flat, n, K, n_r, n_theta, out, nthreads)
+    pf = out.view(np.complex128).reshape(K, n_theta, n_r)
+    return pf.transpose(2, 1, 0)
