@@ -18,8 +18,8 @@
 // matrix E (M x n^2, M = n_r*n_theta/2) with the real image stack (n^2 x K):
 // FP64 MFMA (v_mfma_f64_16x16x4f64).  E is never stored: the exponential
 // factors per axis, E(m,(u,v)) = ax(u,m) * ay(v,m), come from two small tables
-// (n x M complex each) and each lane forms its operand with one complex
-// product.  Images are first staged pixel-major (pixel rows of Kp images,
+// and a per-point rotation, and each lane forms its operand with complex
+// products in registers.  Images are first staged pixel-major (pixel rows of Kp images,
 // FP64) so the B fragments load as 128-byte rows.
 #include <hip/hip_runtime.h>
 
@@ -41,21 +41,23 @@ struct Geo {
     int Kc, Kp;                  // images in this chunk, roundup(Kc, KB)
 };
 
-// ax[u*Mp + m] = exp(+1i*fx(m)*g(u)) (u < nup; zero for u >= n or m >= M),
-// ay[v*Mp + m] = exp(+1i*fy(m)*g(v)) (v < n).  grid: (Mp/256, nup + n).
+// ax[u*Mp + m] = exp(+1i*fx(m)*g(u)) (u < 4; the first four rows),
+// ay[v*Mp + m] = exp(+1i*fy(m)*g(v)) (v < n), and
+// ax[4*Mp + m] = exp(+4i*fx(m)), the step from pixel u to u + 4 (zero for
+// m >= M).  grid: (Mp/256, 5 + n).
 __global__ void tables_kernel(Geo g, double2* ax, double2* ay) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (m >= g.Mp) return;
-    const bool is_x = t < g.nup;
-    const int idx = is_x ? t : t - g.nup;
+    const bool is_x = t < 5;
+    const int idx = is_x ? t : t - 5;
     double2 out = make_double2(0.0, 0.0);
-    if (m < g.M && idx < g.n) {
+    if (m < g.M) {
         const int k = m % g.n_r, j = m / g.n_r;
         const double omega0 = 2.0 * M_PI / (2.0 * g.n_r - 1.0);  // cryo_pft.m:49
         const double dtheta = 2.0 * M_PI / g.n_theta;             // cryo_pft.m:50
         const double f = is_x ? (k * omega0) * sin(j * dtheta) : (k * omega0) * cos(j * dtheta);
-        const double gt = ceil(-0.5 * g.n + idx);                 // nudft2.m:40
+        const double gt = idx == 4 && is_x ? 4.0 : ceil(-0.5 * g.n + idx);  // nudft2.m:40
         double s, c;
         sincos(f * gt, &s, &c);
         out = make_double2(c, s);
@@ -101,20 +103,30 @@ __device__ inline void put<float2>(float2* pf, size_t at, double re, double im) 
     pf[at] = make_float2((float)re, (float)im);
 }
 
-// One workgroup: MB polar points x KB images; wave w takes points
+// One workgroup: MB polar points x KB images (tile t of a 1-D grid; the tiles
+// a workgroup's XCD runs are contiguous, polar-point blocks fastest, so they
+// stream the same images through that XCD's L2); wave w takes points
 // [32w, 32w + 32) of the block as two 16-row tiles, each split into its real
 // and imaginary weight rows, against 4 column tiles of 16 images: 16
 // accumulator tiles (16 x 16 FP64 each).  Each step contracts 4 pixels (one
 // MFMA k-block): lane l holds pixel u0 + l/16 for point (l % 16) of a row
-// tile (A) and image (l % 16) of a column tile (B).
+// tile (A) and image (l % 16) of a column tile (B).  Along u the lane's
+// exponential factor advances by the rotation exp(+4i*fx(m)) (one complex
+// product); it restarts from the table at every v.  Staged pixels u >= n
+// are zero, so the factor's value there does not matter.
 template <typename OutT>
 __global__ __launch_bounds__(256, 2) void pft_kernel(Geo g, const double* __restrict__ xt,
                                                   const double2* __restrict__ ax, const double2* __restrict__ ay,
-                                                  OutT* __restrict__ pf, long long k0) {
+                                                  OutT* __restrict__ pf, long long k0, int nmb, int ntiles,
+                                                  int per_xcd) {
+    const int b = blockIdx.x;
+    const int t = (b % 8) * per_xcd + b / 8;  // dispatch goes round-robin over the 8 XCDs
+    if (t >= ntiles) return;
+    const int mb = t % nmb, kb = t / nmb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kq = lane >> 4;
-    const int mA = blockIdx.x * MB + wave * 32 + r16, mB = mA + 16;
-    const int ib = blockIdx.y * KB;
+    const int mA = mb * MB + wave * 32 + r16, mB = mA + 16;
+    const int ib = kb * KB;
     d4 acc[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -122,50 +134,28 @@ __global__ __launch_bounds__(256, 2) void pft_kernel(Geo g, const double* __rest
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int c = 0; c < 4; c++) acc[a][b][c] = d4{0.0, 0.0, 0.0, 0.0};
-    // Steps s = (v, u0): pixels (u0 .. u0+3, v).  The operands of step s+1
-    // are loaded while step s multiplies; the last step reloads its own
-    // (clamped) operands.
-    const int nu = g.nup / 4;
-    const int nsteps = g.n * nu;
-    auto load = [&](int v, int u0, double2& xA, double2& xB, double2& yA, double2& yB, double (&bx)[4]) {
-        const int u = u0 + kq;
-        xA = ax[(size_t)u * g.Mp + mA];
-        xB = ax[(size_t)u * g.Mp + mB];
-        yA = ay[(size_t)v * g.Mp + mA];
-        yB = ay[(size_t)v * g.Mp + mB];
-        const double* xr = xt + ((size_t)v * g.nup + u) * g.Kp + ib + r16;
+    const double2 rA = ax[(size_t)4 * g.Mp + mA], rB = ax[(size_t)4 * g.Mp + mB];
+    const double2 x0A = ax[(size_t)kq * g.Mp + mA], x0B = ax[(size_t)kq * g.Mp + mB];
+    auto cmul = [](double2 p, double2 q) { return make_double2(p.x * q.x - p.y * q.y, p.x * q.y + p.y * q.x); };
+    for (int v = 0; v < g.n; v++) {
+        const double2 yA = ay[(size_t)v * g.Mp + mA], yB = ay[(size_t)v * g.Mp + mB];
+        double2 xA = x0A, xB = x0B;
+        const double* xr = xt + ((size_t)v * g.nup + kq) * g.Kp + ib + r16;
+        for (int u0 = 0; u0 < g.nup; u0 += 4) {
+            double bx[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) bx[c] = xr[16 * c];
-    };
-    double2 xA, xB, yA, yB;
-    double bx[4];
-    load(0, 0, xA, xB, yA, yB, bx);
-    int v = 0, u0 = 0;
-    for (int s = 0; s < nsteps; s++) {
-        int vn = v, un = u0 + 4;
-        if (un == g.nup) { un = 0; vn++; }
-        if (s + 1 == nsteps) { vn = v; un = u0; }
-        double2 nxA, nxB, nyA, nyB;
-        double nbx[4];
-        load(vn, un, nxA, nxB, nyA, nyB, nbx);
-        // E = ax * ay (complex)
-        const double eAr = xA.x * yA.x - xA.y * yA.y, eAi = xA.x * yA.y + xA.y * yA.x;
-        const double eBr = xB.x * yB.x - xB.y * yB.y, eBi = xB.x * yB.y + xB.y * yB.x;
+            for (int c = 0; c < 4; c++) bx[c] = xr[(size_t)u0 * g.Kp + 16 * c];
+            const double2 eA = cmul(xA, yA), eB = cmul(xB, yB);  // E = ax * ay
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            acc[0][0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eAr, bx[c], acc[0][0][c], 0, 0, 0);
-            acc[0][1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eAi, bx[c], acc[0][1][c], 0, 0, 0);
-            acc[1][0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eBr, bx[c], acc[1][0][c], 0, 0, 0);
-            acc[1][1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eBi, bx[c], acc[1][1][c], 0, 0, 0);
+            for (int c = 0; c < 4; c++) {
+                acc[0][0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eA.x, bx[c], acc[0][0][c], 0, 0, 0);
+                acc[0][1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eA.y, bx[c], acc[0][1][c], 0, 0, 0);
+                acc[1][0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eB.x, bx[c], acc[1][0][c], 0, 0, 0);
+                acc[1][1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(eB.y, bx[c], acc[1][1][c], 0, 0, 0);
+            }
+            xA = cmul(xA, rA);
+            xB = cmul(xB, rB);
         }
-        xA = nxA;
-        xB = nxB;
-        yA = nyA;
-        yB = nyB;
-#pragma unroll
-        for (int c = 0; c < 4; c++) bx[c] = nbx[c];
-        v = vn;
-        u0 = un;
     }
     // D tile layout of the FP64 16x16x4 MFMA: lane l, element t holds row
     // 4*t + l/16, column l % 16 (one row per element, unlike the FP32 tiles).
@@ -173,8 +163,8 @@ __global__ __launch_bounds__(256, 2) void pft_kernel(Geo g, const double* __rest
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int m = blockIdx.x * MB + wave * 32 + a * 16 + 4 * t + kq;
+        for (int e = 0; e < 4; e++) {
+            const int m = mb * MB + wave * 32 + a * 16 + 4 * e + kq;
             if (m >= g.M) continue;
             const int k = m % g.n_r, j = m / g.n_r;
 #pragma unroll
@@ -182,7 +172,7 @@ __global__ __launch_bounds__(256, 2) void pft_kernel(Geo g, const double* __rest
                 const int i = ib + 16 * c + r16;
                 if (i >= g.Kc) continue;
                 const size_t img = (size_t)(k0 + i) * g.n_theta;
-                const double re = acc[a][0][c][t], im = acc[a][1][c][t];
+                const double re = acc[a][0][c][e], im = acc[a][1][c][e];
                 put<OutT>(pf, (img + j) * g.n_r + k, re, im);
                 put<OutT>(pf, (img + j + L) * g.n_r + k, re, -im);
             }
@@ -222,13 +212,13 @@ int pft_run(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, 
     const int kc = K < KCHUNK ? K : KCHUNK;
     const int kcp = (kc + KB - 1) / KB * KB;
     int rc;
-    if ((rc = ensure(c, c->pft_ax, (size_t)g.nup * g.Mp * sizeof(double2))) ||
+    if ((rc = ensure(c, c->pft_ax, (size_t)5 * g.Mp * sizeof(double2))) ||
         (rc = ensure(c, c->pft_ay, (size_t)g.n * g.Mp * sizeof(double2))) ||
         (rc = ensure(c, c->pft_xt, (size_t)g.n * g.nup * (size_t)kcp * sizeof(double))))
         return rc;
     hipEvent_t* ev = c->ev;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    tables_kernel<<<dim3(g.Mp / 256 + (g.Mp % 256 != 0), g.nup + g.n), 256, 0, c->stream>>>(
+    tables_kernel<<<dim3(g.Mp / 256 + (g.Mp % 256 != 0), 5 + g.n), 256, 0, c->stream>>>(
         g, (double2*)c->pft_ax.p, (double2*)c->pft_ay.p);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
@@ -246,13 +236,17 @@ int pft_run(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, 
             stage_kernel<float><<<sgrid, 256, 0, c->stream>>>(g, (const float*)d_img, k0, K, (double*)c->pft_xt.p);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
-        dim3 grid(g.Mp / MB, g.Kp / KB);
+        const int nmb = g.Mp / MB, ntiles = nmb * (g.Kp / KB);
+        const int per = (ntiles + 7) / 8;
+        const dim3 grid(8 * per);
         if (p->pf_dtype == CLM_C128)
             pft_kernel<double2><<<grid, 256, 0, c->stream>>>(g, (const double*)c->pft_xt.p, (const double2*)c->pft_ax.p,
-                                                               (const double2*)c->pft_ay.p, (double2*)d_pf, k0);
+                                                               (const double2*)c->pft_ay.p, (double2*)d_pf, k0, nmb,
+                                                               ntiles, per);
         else
             pft_kernel<float2><<<grid, 256, 0, c->stream>>>(g, (const double*)c->pft_xt.p, (const double2*)c->pft_ax.p,
-                                                              (const double2*)c->pft_ay.p, (float2*)d_pf, k0);
+                                                              (const double2*)c->pft_ay.p, (float2*)d_pf, k0, nmb,
+                                                              ntiles, per);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventRecord(ev[4], c->stream));
         HIPCHK(c, hipEventSynchronize(ev[4]));

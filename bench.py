@@ -432,6 +432,10 @@ def pft_bench(args):
     sec = float(np.mean(times))
     k_ms = float(np.mean(kms))
     achieved = W_img * K / (k_ms * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pft_kernel_k5000_traffic.json")
+    if os.path.exists(tpath) and K == 5000:
+        traffic = json.load(open(tpath))["hbm_bytes_per_launch"]
     Mp = (M + 127) // 128 * 128
     nup = (n + 3) // 4 * 4
     Kp = (K + 63) // 64 * 64
@@ -459,7 +463,7 @@ def pft_bench(args):
            "config": {"workload": "cryo_pft of the K=5000 common-lines workload's images", "K": K, "n": n,
                       "n_r": n_r, "n_theta": T},
            "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": 78.6, "unit": "TFLOP/s",
-                        "frac": achieved / 78.6e12, "traffic": None, "kernel": "pft_kernel", "kernel_ms": k_ms,
+                        "frac": achieved / 78.6e12, "traffic": traffic, "kernel": "pft_kernel", "kernel_ms": k_ms,
                         "flops_per_image": W_img, "executed_flops": 4.0 * Mp * n * nup * Kp},
            "stats": st,
            "cpu_baseline": {"value": ns / cpu_sec, "unit": "images/s", "cores": nth, "kind": "port",
