@@ -29,7 +29,7 @@ EXPORTS = (
     "clm_clmatrix_grouped", "clm_autotune", "clm_clmatrix_windowed",
     "clm_shiftnet_nparam", "clm_shiftnet_features", "clm_shiftnet_forward",
     "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
-    "clm_pft_device", "clm_pft", "clm_pft_freqs",
+    "clm_pft_device", "clm_pft", "clm_pft_freqs", "clm_estimate_shifts",
 )
 
 CLM_GROUP_GRAPH = 1
@@ -133,6 +133,20 @@ class ClmPftStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ClmLsqrStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("flag", ctypes.c_int32),
+        ("relres", ctypes.c_double),
+        ("normar", ctypes.c_double),
+        ("norm_a", ctypes.c_double),
+        ("ms", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -197,6 +211,9 @@ def load():
     L.clm_pft_device.argtypes = [vp, ctypes.POINTER(ClmPftParams), vp, vp, ctypes.POINTER(ClmPftStats)]
     L.clm_pft.restype = ctypes.c_int
     L.clm_pft.argtypes = [vp, ctypes.POINTER(ClmPftParams), vp, vp, ctypes.POINTER(ClmPftStats)]
+    L.clm_estimate_shifts.restype = ctypes.c_int
+    L.clm_estimate_shifts.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.c_double,
+                                      ctypes.c_int32, ctypes.c_double, vp, ctypes.POINTER(ClmLsqrStats)]
     L.clm_pft_freqs.restype = ctypes.c_int
     L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L

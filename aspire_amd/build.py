@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 # (source, extra flags): the engine, the shift network, and the pair-kernel
 # instantiations, one object per (operand type, shift-grid symmetry)
 UNITS = [(os.path.join(CSRC, "clm_api.hip"), []), (os.path.join(CSRC, "shiftnet.hip"), []),
-         (os.path.join(CSRC, "pft.hip"), [])] + [
+         (os.path.join(CSRC, "pft.hip"), []), (os.path.join(CSRC, "shifts.hip"), [])] + [
     (os.path.join(CSRC, "pair_variants.hip"), [f"-DCLM_VARIANT_OP={op}", f"-DCLM_VARIANT_SYM={sym}"])
     for op in ("bf16", "f16", "f32") for sym in (1, 0)]
 ARCH = "gfx950"

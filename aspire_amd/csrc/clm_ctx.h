@@ -54,6 +54,8 @@ struct clm_ctx {
     clmh::DevBuf sn_h1, sn_h2, sn_z, sn_y, sn_dy, sn_dz, sn_dz2, sn_dz1;
     // polar Fourier transform: per-axis tables, staged images, host-form buffers
     clmh::DevBuf pft_ax, pft_ay, pft_xt, pft_img, pft_out;
+    // shift least squares: packed lines, u, row partial sums, 2K vectors, tables, scalar state
+    clmh::DevBuf ls_pack, ls_u, ls_part, ls_vec, ls_tab, ls_state;
     // autotuned variants: shape key -> (ct, W, lag)
     std::vector<std::pair<unsigned long long, int4>> tuned;
     clmh::DevBuf tune_tiles;

@@ -50,12 +50,13 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic variant)")
     ap.add_argument("--tune-pairs", type=int, default=200000)
     ap.add_argument("--flags", type=int, default=0, help="clm_params.reserved bits (1: no lagged schedule)")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped", "configs", "pft"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped", "configs", "pft", "shifts"],
                     help="pairs: one BF16 call on the workload and nothing else (for counter "
                          "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration); "
                          "grouped: the 64-stack uneven batch in one graph-captured grouped job; "
                          "configs: the K=100 and K=1000 workloads in every precision against the oracle; "
-                         "pft: the polar Fourier transform of the K=5000 workload's images")
+                         "pft: the polar Fourier transform of the K=5000 workload's images; "
+                         "shifts: the LSQR solve of the K=5000 workload's shift equations")
     ap.add_argument("--stacks", type=int, default=64, help="grouped mode: number of stacks")
     ap.add_argument("--grouped-sample", type=int, default=40, help="grouped mode: oracle pairs per stack")
     args = ap.parse_args()
@@ -72,6 +73,8 @@ def main():
         return configs_bench(args)
     if args.mode == "pft":
         return pft_bench(args)
+    if args.mode == "shifts":
+        return shifts_bench(args)
 
     if args.mode == "copy":
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -470,6 +473,69 @@ def pft_bench(args):
                             "sample": f"{ns} images of the workload, FP64 C oracle (OpenMP)"},
            "parity_sample": {"images": ns, "max_abs_err_rel_to_max": rel},
            "torch_zgemm": {"seconds": t_torch, "images_per_s": K / t_torch, "max_diff_rel_to_max": zdiff}}
+    print(json.dumps(out))
+
+
+def shifts_bench(args):
+    """2-D shifts by LSQR (tol 1e-8, maxit 100: cryo_estimate_shifts.m:229)
+    on the shift equations of the K=5000 workload, computed on the device
+    from the BF16 engine's per-pair outputs.  HBM-bound: the algorithmic
+    traffic is 32 bytes per pair per iteration (packed lines read by A*v
+    and A'*u, u read twice and written once).  CPU baseline: the oracle's
+    LSQR (scipy CSR products) on the same full system, a few iterations
+    timed and scaled to the iterations the device ran."""
+    import torch
+
+    from aspire_amd import sim
+    from aspire_amd.clmatrix import clmatrix_records
+    from aspire_amd.shifts import estimate_shifts
+    from oracle import shifts_oracle
+
+    K, n_theta = args.K, 360
+    st0 = sim.make_stack(K=K, n=129, n_r=65, n_theta=n_theta, snr=1 / 16, max_shift_2d=5.0, shift_kind="uniform",
+                         seed=1003, device="cuda")
+    rec, _ = clmatrix_records(st0.pf, 10, 0.5, precision="bf16")
+    del st0
+    torch.cuda.synchronize()
+    P = K * (K - 1) // 2
+    for i in range(args.warmup):
+        x, st = estimate_shifts(rec["cl1"], rec["cl2"], rec["shift"], K, n_theta)
+    times, sts = [], []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x, st = estimate_shifts(rec["cl1"], rec["cl2"], rec["shift"], K, n_theta)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        sts.append(st)
+        log(f"shifts step {i}: {times[-1]:.4f}s {st}")
+    sec = float(np.mean(times))
+    it = st["iterations"]
+    dev_ms = float(np.mean([s_["ms"] for s_ in sts]))
+    achieved = 32.0 * P * it / (dev_ms * 1e-3)
+    # CPU oracle: same system, a few iterations
+    cl1, cl2, b = (rec[k].cpu().numpy() for k in ("cl1", "cl2", "shift"))
+    A, bb = shifts_oracle.system(K, n_theta, cl1, cl2, b)
+    n_cpu = 3
+    t = time.perf_counter()
+    xo, io = shifts_oracle.lsqr(A, bb, tol=0.0, maxit=n_cpu)
+    cpu_it = (time.perf_counter() - t) / n_cpu
+    xg, _ = estimate_shifts(rec["cl1"], rec["cl2"], rec["shift"], K, n_theta, tol=0.0, maxit=n_cpu)
+    xg = xg.cpu().numpy().ravel()
+    out = {"metric": "pairs/s: LSQR of the shift equations, K=5000 (cryo_estimate_shifts.m:229)",
+           "value": P / sec, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * sec, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "shift equations of the K=5000 common-lines workload", "K": K, "pairs": P,
+                      "tol": 1e-8, "maxit": 100},
+           "lsqr": st,
+           "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
+                        "frac": achieved / 8e12, "traffic": None, "kernel": "whole solve",
+                        "bytes_per_pair_iteration": 32},
+           "cpu_baseline": {"value": P / (cpu_it * it), "unit": "pairs/s", "cores": 1, "kind": "port",
+                            "sample": f"{n_cpu} LSQR iterations of the oracle (scipy CSR) on the full system, "
+                                      f"scaled to {it} iterations"},
+           "parity_sample": {"iterations": n_cpu,
+                             "max_abs_diff_rel_to_max": float(np.abs(xg - xo).max() / np.abs(xo).max())}}
     print(json.dumps(out))
 
 

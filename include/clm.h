@@ -321,6 +321,35 @@ int clm_pft(clm_ctx *ctx, const clm_pft_params *p, const void *img, void *pf,
 /* freqs of cryo_pft.m:52-58, (n_r*n_theta/2) x 2 column-major doubles. */
 int clm_pft_freqs(int32_t n_r, int32_t n_theta, double *freqs);
 
+/*
+ * 2-D shifts from the shift equations, on the device: the step after
+ * cryo_clmatrix.  Replaces
+ *   est_shifts = lsqr(shift_equations(:,1:end-1), shift_equations(:,end), tol, maxit)
+ * of sinograms/cryo_estimate_shifts.m:229-230 (tol 1e-8, maxit 100 there).
+ * The matrix is not formed: its rows (cryo_clmatrix_cpu.m:722-741) are
+ * evaluated from the per-pair outputs of clm_clmatrix_device (d_cl1, d_cl2,
+ * d_shift1d, pair order k1 < k2).  LSQR (Paige and Saunders 1982) from
+ * x0 = 0 with damping `damp` (1 gives the eye(2K) block of
+ * sinograms/test_shifts.m:24-27, without its three gauge rows), stopping
+ * when ||b - Ax|| <= tol ||b|| or ||A'(b - Ax)|| <= tol ||A||_F ||b - Ax||
+ * (running estimates), or after maxit iterations.  A pair whose line
+ * numbers lie outside [0, n_theta] contributes a zero row.
+ * d_x (device, 2K doubles): est_shifts(k,:) = [d_x[2k], d_x[2k+1]].
+ */
+typedef struct clm_lsqr_stats {
+    int32_t iterations; /* LSQR iterations run                              */
+    int32_t flag;       /* 0: a stopping test was met; 1: maxit reached     */
+    double relres;      /* estimate of ||b - Ax|| / ||b||                   */
+    double normar;      /* estimate of ||A'(b - Ax)|| / (||A||_F ||b - Ax||) */
+    double norm_a;      /* running Frobenius-norm estimate of A             */
+    float ms;           /* device time of the solve                         */
+} clm_lsqr_stats;
+
+int clm_estimate_shifts(clm_ctx *ctx, int32_t K, int32_t n_theta,
+                        const int32_t *d_cl1, const int32_t *d_cl2,
+                        const double *d_shift1d, double tol, int32_t maxit,
+                        double damp, double *d_x, clm_lsqr_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
