@@ -36,6 +36,8 @@ struct State {
 
 __device__ inline long long first(long long k1, long long K) { return k1 * K - k1 * (k1 + 1) / 2; }
 
+constexpr int NS = 16;  // k1 residue classes of the column part of A'u (partial sums)
+
 // ta[c] = (sin a, cos a), a = (c-1)*pi/L; tb[c] = (-sin b', -cos b', sign):
 // b = (c-1)*pi/L, b' = b or b - pi, sign = +1 (b < pi) or -1 (b >= pi), for
 // line numbers c = 0..2L (0 = no line; cryo_clmatrix_cpu.m:722-741).  Entry
@@ -153,44 +155,44 @@ __global__ __launch_bounds__(1024) void beta_kernel(int n, const double* part, S
     }
 }
 
-// vt_row[2k..2k+1] = sum over pairs (k, k2 > k) of the k1 coefficients * u.
-__global__ __launch_bounds__(256) void atu_row_kernel(int K, const unsigned* pack, const double2* ta,
-                                                      const double* tb, const double* u, const State* S,
-                                                      double* vt_row) {
-    __shared__ double sh[256];
-    if (S->done) return;
-    const int k = blockIdx.x;
-    const long long e0 = first(k, K);
-    const int n = K - 1 - k;
-    double s0 = 0.0, s1 = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) {
-        const unsigned pk = pack[e0 + j];
-        const double2 A = ta[pk & 0xffff];
-        const double sg = tb[3 * (pk >> 16) + 2];
-        const double ue = u[e0 + j];
-        s0 += sg * A.x * ue;
-        s1 += sg * A.y * ue;
-    }
-    s0 = block_sum<256>(s0, sh);
-    s1 = block_sum<256>(s1, sh);
-    if (threadIdx.x == 0) {
-        vt_row[2 * k] = s0;
-        vt_row[2 * k + 1] = s1;
-    }
-}
-
-// vt_col[2k..2k+1] = sum over pairs (k1 < k, k) of the k2 coefficients * u.
-// A block takes 16 consecutive images: thread (image i, lane q) visits
-// k1 = q, q + 16, ...; for one k1 the 16 images' pairs are consecutive.
-__global__ __launch_bounds__(256) void atu_col_kernel(int K, const unsigned* pack, const double* tb,
-                                                      const double* u, const State* S, double* vt_col) {
+// A'u in one launch, in fixed-order partial sums:
+//  blocks [0, K): vt_row[2k..2k+1] = sum over pairs (k, k2 > k) of the k1
+//    coefficients * u (one image, its pairs contiguous);
+//  blocks [K, K + ceil(K/16)*NS): vt_col[s][2k..2k+1] = sum over pairs
+//    (k1 < k, k), k1 = s*16 + q (mod 16*NS), of the k2 coefficients * u: 16
+//    consecutive images per block, so for one k1 the 16 pairs are
+//    consecutive; lane q of image i walks its residue class.
+__global__ __launch_bounds__(256) void atu_kernel(int K, const unsigned* pack, const double2* ta, const double* tb,
+                                                  const double* u, const State* S, double* vt_row, double* vt_col) {
     __shared__ double sh0[256], sh1[256];
     if (S->done) return;
-    const int i = threadIdx.x & 15, q = threadIdx.x >> 4;
-    const int k = blockIdx.x * 16 + i;
     double s0 = 0.0, s1 = 0.0;
+    if ((int)blockIdx.x < K) {
+        const int k = blockIdx.x;
+        const long long e0 = first(k, K);
+        const int n = K - 1 - k;
+        for (int j = threadIdx.x; j < n; j += 256) {
+            const unsigned pk = pack[e0 + j];
+            const double2 A = ta[pk & 0xffff];
+            const double sg = tb[3 * (pk >> 16) + 2];
+            const double ue = u[e0 + j];
+            s0 += sg * A.x * ue;
+            s1 += sg * A.y * ue;
+        }
+        s0 = block_sum<256>(s0, sh0);
+        s1 = block_sum<256>(s1, sh0);
+        if (threadIdx.x == 0) {
+            vt_row[2 * k] = s0;
+            vt_row[2 * k + 1] = s1;
+        }
+        return;
+    }
+    const int cb = blockIdx.x - K;
+    const int sp = cb % NS, kb = cb / NS;
+    const int i = threadIdx.x & 15, q = threadIdx.x >> 4;
+    const int k = kb * 16 + i;
     if (k < K) {
-        for (int k1 = q; k1 < k; k1 += 16) {
+        for (int k1 = sp * 16 + q; k1 < k; k1 += 16 * NS) {
             const long long e = first(k1, K) + (k - k1 - 1);
             const unsigned pk = pack[e];
             const double* B = tb + 3 * (pk >> 16);
@@ -208,9 +210,16 @@ __global__ __launch_bounds__(256) void atu_col_kernel(int K, const unsigned* pac
             a0 += sh0[r * 16 + i];
             a1 += sh1[r * 16 + i];
         }
-        vt_col[2 * k] = a0;
-        vt_col[2 * k + 1] = a1;
+        vt_col[(size_t)sp * 2 * K + 2 * k] = a0;
+        vt_col[(size_t)sp * 2 * K + 2 * k + 1] = a1;
     }
+}
+
+__device__ inline double atu_value(int n2, const double* vt_row, const double* vt_col, int i) {
+    double t = vt_row[i];
+#pragma unroll
+    for (int sp = 0; sp < NS; sp++) t += vt_col[(size_t)sp * n2 + i];
+    return t;
 }
 
 // First direction: v = A'u / |A'u| with u = b/beta; w = v; x = 0.
@@ -224,7 +233,7 @@ __global__ __launch_bounds__(1024) void vinit_kernel(int n2, const double* vt_ro
     const double ib = 1.0 / S->beta;
     double s = 0.0;
     for (int i = threadIdx.x; i < n2; i += 1024) {
-        const double t = (vt_row[i] + vt_col[i]) * ib;
+        const double t = atu_value(n2, vt_row, vt_col, i) * ib;
         v[i] = t;
         s += t * t;
     }
@@ -268,7 +277,7 @@ __global__ __launch_bounds__(1024) void step_kernel(int n2, const double* vt_row
         const double ib = 1.0 / beta;
         double s = 0.0;
         for (int i = threadIdx.x; i < n2; i += 1024) {
-            const double t = (vt_row[i] + vt_col[i]) * ib - beta * v[i];
+            const double t = atu_value(n2, vt_row, vt_col, i) * ib - beta * v[i];
             v[i] = t;
             s += t * t;
         }
@@ -335,11 +344,11 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     const int L = n_theta / 2, n2 = 2 * K;
     int rc;
     if ((rc = ensure(c, c->ls_pack, (size_t)P * 4)) || (rc = ensure(c, c->ls_u, (size_t)P * 8)) ||
-        (rc = ensure(c, c->ls_part, (size_t)K * 8)) || (rc = ensure(c, c->ls_vec, (size_t)n2 * 4 * 8)) ||
+        (rc = ensure(c, c->ls_part, (size_t)K * 8)) || (rc = ensure(c, c->ls_vec, (size_t)n2 * (3 + NS) * 8)) ||
         (rc = ensure(c, c->ls_tab, (size_t)(2 * L + 2) * 5 * 8)) || (rc = ensure(c, c->ls_state, sizeof(State))))
         return rc;
     double* vec = (double*)c->ls_vec.p;
-    double *v = vec, *w = vec + n2, *vt_row = vec + 2 * n2, *vt_col = vec + 3 * n2;
+    double *v = vec, *w = vec + n2, *vt_row = vec + 2 * n2, *vt_col = vec + 3 * n2;  // vt_col: NS x 2K
     double2* ta = (double2*)c->ls_tab.p;
     double* tb = (double*)c->ls_tab.p + 2 * (2 * L + 2);
     State* S = (State*)c->ls_state.p;
@@ -357,16 +366,24 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     tables_kernel<<<(2 * L + 2 + 255) / 256, 256, 0, c->stream>>>(L, ta, tb);
     init_kernel<<<K, 256, 0, c->stream>>>(K, L, d_cl1, d_cl2, d_shift1d, pack, u, part);
     beta_kernel<<<1, 1024, 0, c->stream>>>(K, part, S, 1);
-    atu_row_kernel<<<K, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row);
-    atu_col_kernel<<<(K + 15) / 16, 256, 0, c->stream>>>(K, pack, tb, u, S, vt_col);
+    const int natu = K + (K + 15) / 16 * NS;
+    atu_kernel<<<natu, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row, vt_col);
     vinit_kernel<<<1, 1024, 0, c->stream>>>(n2, vt_row, vt_col, S, v, w, d_x);
     HIPCHK(c, hipGetLastError());
+    // launches after the stopping tests pass return at once; the done flag is
+    // read back every 8 iterations so a converged solve stops issuing them
     for (int it = 0; it < maxit; it++) {
         av_kernel<<<K, 256, 0, c->stream>>>(K, pack, ta, tb, v, S, u, part);
         beta_kernel<<<1, 1024, 0, c->stream>>>(K, part, S, 0);
-        atu_row_kernel<<<K, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row);
-        atu_col_kernel<<<(K + 15) / 16, 256, 0, c->stream>>>(K, pack, tb, u, S, vt_col);
+        atu_kernel<<<natu, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row, vt_col);
         step_kernel<<<1, 1024, 0, c->stream>>>(n2, vt_row, vt_col, S, v, w, d_x);
+        if (it % 8 == 7 && it + 1 < maxit) {
+            int done = 0;
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(&done, &S->done, sizeof done, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (done) break;
+        }
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&h, S, sizeof h, hipMemcpyDeviceToHost, c->stream));
