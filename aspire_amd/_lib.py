@@ -30,6 +30,7 @@ EXPORTS = (
     "clm_shiftnet_nparam", "clm_shiftnet_features", "clm_shiftnet_forward",
     "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
     "clm_pft_device", "clm_pft", "clm_pft_freqs", "clm_estimate_shifts",
+    "clm_clmatrix_filtered", "clm_clmatrix_filtered_device",
 )
 
 CLM_GROUP_GRAPH = 1
@@ -180,6 +181,11 @@ def load():
     L.clm_clmatrix_device.restype = ctypes.c_int
     L.clm_clmatrix_device.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(ClmStats)]
+    for name in ("clm_clmatrix_filtered", "clm_clmatrix_filtered_device"):
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [vp, ctypes.POINTER(ClmParams), ctypes.c_int32, vp, vp, vp, vp, vp, vp,
+                       ctypes.POINTER(ClmStats)]
     L.clm_clmatrix_grouped.restype = ctypes.c_int
     L.clm_clmatrix_grouped.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ClmParams),
                                        ctypes.POINTER(ClmStackIO), ctypes.c_int32,

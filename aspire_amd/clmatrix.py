@@ -59,7 +59,7 @@ def _as_matlab_pf(pf: np.ndarray):
 
 
 def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=0,
-                     tie_margin=1e-3, variant=0, verbose=0, flags=0):
+                     tie_margin=1e-3, variant=0, verbose=0, flags=0, map_filter_radius=0):
     """Per-pair records of the common-lines search.
 
     ``pf`` is a numpy array (host path) or a torch tensor on a HIP device of
@@ -68,7 +68,9 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
     Returns ``(records, stats)``: records hold per-pair ``cl1``, ``cl2``
     (1-based), ``j`` (0-based shift index, -1 = none), ``corr`` (best
     correlation) and ``shift``, in equation order; numpy arrays for numpy
-    input, device tensors for tensor input.
+    input, device tensors for tensor input.  ``map_filter_radius`` d > 0
+    filters every map with cryo_average_clmap before the search (FP64 path,
+    ``clm_clmatrix_filtered``); ``corr`` is then the filtered value.
     """
     ctx = _lib.context(device)
     stats = _lib.ClmStats()
@@ -87,11 +89,15 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
                     flags)
         # torch's own queue must be done with pf before libclm's stream reads it
         torch.cuda.current_stream(dev).synchronize()
-        rc = _lib.load().clm_clmatrix_device(
-            ctx.handle, ctypes.byref(p), ctypes.c_void_p(base.data_ptr()),
-            ctypes.c_void_p(out["cl1"].data_ptr()), ctypes.c_void_p(out["cl2"].data_ptr()),
-            ctypes.c_void_p(out["j"].data_ptr()), ctypes.c_void_p(out["corr"].data_ptr()),
-            ctypes.c_void_p(out["shift"].data_ptr()), ctypes.byref(stats))
+        args = (ctypes.c_void_p(base.data_ptr()),
+                ctypes.c_void_p(out["cl1"].data_ptr()), ctypes.c_void_p(out["cl2"].data_ptr()),
+                ctypes.c_void_p(out["j"].data_ptr()), ctypes.c_void_p(out["corr"].data_ptr()),
+                ctypes.c_void_p(out["shift"].data_ptr()), ctypes.byref(stats))
+        if map_filter_radius:
+            rc = _lib.load().clm_clmatrix_filtered_device(ctx.handle, ctypes.byref(p), int(map_filter_radius),
+                                                          *args)
+        else:
+            rc = _lib.load().clm_clmatrix_device(ctx.handle, ctypes.byref(p), *args)
         ctx.check(rc)
         return out, stats.as_dict()
     pfm, dt = _as_matlab_pf(pf)
@@ -101,11 +107,14 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="bf16", device=
                corr=np.zeros(P, np.float64), shift=np.zeros(P, np.float64))
     p = _params(n_r, n_theta, K, dt, max_shift, shift_step, precision, verbose, tie_margin, variant,
                 flags)
-    rc = _lib.load().clm_clmatrix(
-        ctx.handle, ctypes.byref(p), pfm.ctypes.data_as(ctypes.c_void_p),
-        out["cl1"].ctypes.data_as(ctypes.c_void_p), out["cl2"].ctypes.data_as(ctypes.c_void_p),
-        out["j"].ctypes.data_as(ctypes.c_void_p), out["corr"].ctypes.data_as(ctypes.c_void_p),
-        out["shift"].ctypes.data_as(ctypes.c_void_p), ctypes.byref(stats))
+    args = (pfm.ctypes.data_as(ctypes.c_void_p),
+            out["cl1"].ctypes.data_as(ctypes.c_void_p), out["cl2"].ctypes.data_as(ctypes.c_void_p),
+            out["j"].ctypes.data_as(ctypes.c_void_p), out["corr"].ctypes.data_as(ctypes.c_void_p),
+            out["shift"].ctypes.data_as(ctypes.c_void_p), ctypes.byref(stats))
+    if map_filter_radius:
+        rc = _lib.load().clm_clmatrix_filtered(ctx.handle, ctypes.byref(p), int(map_filter_radius), *args)
+    else:
+        rc = _lib.load().clm_clmatrix(ctx.handle, ctypes.byref(p), *args)
     ctx.check(rc)
     return out, stats.as_dict()
 
@@ -237,18 +246,19 @@ def cryo_clmatrix(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, map_filter
 
     Arguments and defaults follow cryo_clmatrix_cpu.m:123-152: ``NK`` is
     accepted and ignored (:123), ``max_shift`` defaults to 15, ``shift_step``
-    to 1.  ``map_filter_radius`` > 0 (argument 6) is not supported by the
-    GPU engine and raises; ``ref_clmatrix``/``ref_shifts_2d`` only feed the
-    reference's debug output and are ignored.  ``precision`` selects the
-    arithmetic: "bf16" (default), "fp16", "fp32" or "fp64".
+    to 1.  ``map_filter_radius`` > 0 (argument 6, :24-29) filters every map
+    with cryo_average_clmap before the search (FP64 path; corrstack then
+    holds 1 - the filtered value, as the reference stores it);
+    ``ref_clmatrix``/``ref_shifts_2d`` only feed the reference's debug output
+    and are ignored.  ``precision`` selects the arithmetic of the unfiltered
+    search: "bf16" (default), "fp16", "fp32" or "fp64".
     """
-    if map_filter_radius:
-        raise NotImplementedError("map_filter_radius > 0 is not implemented by the HIP engine")
     if hasattr(pf, "shape") and len(pf.shape) >= 2 and pf.shape[1] % 2:
         raise ValueError("n_theta must be even")  # cryo_clmatrix_cpu.m:84-86
     if verbose:
         log.info("Shift estimation parameters: max_shift=%g   shift_step=%g", max_shift, shift_step)
-    rec, stats = clmatrix_records(pf, max_shift, shift_step, precision, device, verbose=verbose)
+    rec, stats = clmatrix_records(pf, max_shift, shift_step, precision, device, verbose=verbose,
+                                  map_filter_radius=map_filter_radius)
     if type(pf).__module__.startswith("torch"):
         rec = {k: v.cpu().numpy() for k, v in rec.items()}
         n_theta, K = pf.shape[1], (pf.shape[2] if pf.dim() == 3 else 1)

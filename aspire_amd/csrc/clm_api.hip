@@ -3,6 +3,7 @@
 #include "clm.h"
 #include "clm_ctx.h"
 #include "clm_kernels.hip"
+#include "clm_clmap.h"
 
 #include <algorithm>
 #include <cmath>
@@ -192,6 +193,7 @@ struct StackJob {
     const double* lsh = nullptr;     // per-line pre-shift [K][L] (windowed search) or null
     int force64 = 0;                 // FP64 near-tie re-score even for FP32 operands (refinement)
     int twin_src = -1;               // FP32 twin: table index of the stack it refines
+    int filt = 0;                    // map_filter_radius (> 0: FP64 search of filtered maps)
     // selected pairs only (device-built): list of (stack, slot), count, pass-1 tiles
     const SlotRef* sel_list = nullptr;
     const int* sel_count = nullptr;
@@ -204,6 +206,7 @@ struct StackJob {
 struct Group {
     int opk = 0, exact = 0, ct = 0, Rp = 0, sym = 0, W = 0, CPP = 0, lag = 0;
     int sel = 0;                // one stack of selected pairs (device-built tiles and list)
+    int filt = 0;               // (FP64 groups) map_filter_radius of its stacks
     int norefine = 0;           // no FP32 twin: uncertain pairs take the second operand-type pass
     int twin = -1;              // index of the FP32 twin group refining this group's uncertain pairs
     int twin_of = -1;           // (twin groups) the group it refines
@@ -290,6 +293,14 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
             if (n > 1) c->err = "stack " + std::to_string(i) + ": " + c->err;
             return rc;
         }
+        if (j.filt > 0) {  // filtered maps: the FP64 search of every entry
+            // cryo_average_clmap's fft2(W, 2L, 2L) would truncate a wider weight square
+            if (2 * j.filt + 1 > 2 * j.s.g.L)
+                return fail(c, CLM_E_UNSUP, "map_filter_radius %d: the (2d+1)-square exceeds the 2L = %d map", j.filt,
+                            2 * j.s.g.L);
+            j.exact = 1;
+            j.use64 = 1;
+        }
         if (j.P > 0 && !j.io.pf) return fail(c, CLM_E_ARG, "null pf (stack %d)", i);
     }
     // groups: same kernel instantiation (pairs-free stacks need none)
@@ -302,7 +313,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         int gi = -1;
         for (size_t q = 0; q < pl.groups.size() && !j.sel_list; q++) {
             const Group& G = pl.groups[q];
-            if (!G.sel && G.exact == j.exact && G.norefine == norefine &&
+            if (!G.sel && G.exact == j.exact && G.filt == j.filt && G.norefine == norefine &&
                 (j.exact || (G.opk == j.opk && G.ct == j.v.ct && G.Rp == g.Rp && G.sym == g.sym && G.W == j.v.W &&
                              G.CPP == j.v.CPP && G.lag == lag))) {
                 gi = (int)q;
@@ -315,6 +326,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
             G.W = j.v.W; G.CPP = j.v.CPP; G.lag = lag;
             G.sel = j.sel_list != nullptr;
             G.norefine = norefine;
+            G.filt = j.filt;
             pl.groups.push_back(G);
             gi = (int)pl.groups.size() - 1;
         }
@@ -412,6 +424,15 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         if ((rc = upload(c, B.ph64, j.s.ph64))) return rc;
         if ((rc = upload(c, B.shiftv, j.s.shifts))) return rc;
         if ((rc = upload(c, B.H, j.s.H))) return rc;
+        if (j.filt > 0) {
+            // W0 = 1.25.^(21:-1:1); W(i,j) = W0(1 + |i-d| + |j-d|)  (cryo_average_clmap.m:24-26)
+            const int w = 2 * j.filt + 1;
+            std::vector<double> wv((size_t)w * w);
+            for (int a = 0; a < w; a++)
+                for (int b = 0; b < w; b++)
+                    wv[(size_t)a * w + b] = std::pow(1.25, (double)(21 - (std::abs(a - j.filt) + std::abs(b - j.filt))));
+            if ((rc = upload(c, B.W, wv))) return rc;
+        }
         StackDev& d = pl.table[ti];
         d.g = g;
         d.astk = B.astk.p;
@@ -432,6 +453,8 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         d.fp64 = j.use64 || j.force64;
         d.CPP = j.v.CPP;
         d.npairs = j.P;
+        d.filt = j.filt;
+        d.wts = j.filt > 0 ? (const double*)B.W.p : nullptr;
     }
     for (Group& G : pl.groups) {
         long long base = 0;
@@ -624,6 +647,13 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
             if (G.sel) {
                 ea.list = pl.jobs[G.members[0]].sel_list;
                 ea.n_dev = pl.jobs[G.members[0]].sel_count;
+            }
+            if (G.filt > 0) {
+                ClmapArgs ca{dtab, G.first, (int)G.members.size(), G.P};
+                const long long nb = std::min<long long>(G.P, (long long)c->n_cu * 2);
+                hipLaunchKernelGGL(clmap_kernel, dim3((unsigned)nb), dim3(256), 0, c->stream, ca);
+                HIPCHK(c, hipGetLastError());
+                continue;
             }
             int S = 1;
             for (int i : G.members) S = std::max(S, pl.jobs[i].s.g.S);
@@ -869,7 +899,8 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
     add(pl.table.data(), pl.table.size() * sizeof(StackDev));
     for (size_t ti = 0; ti < pl.order.size(); ti++) {
         const StackBufs& B = c->sb[pl.sb_base + ti];
-        const void* sp[] = {B.P64.p, B.P32.p, B.Bop.p, B.nrm.p, B.astk.p, B.recs.p, B.ph32.p, B.ph64.p, B.shiftv.p, B.H.p};
+        const void* sp[] = {B.P64.p, B.P32.p, B.Bop.p, B.nrm.p, B.astk.p, B.recs.p, B.ph32.p, B.ph64.p, B.shiftv.p, B.H.p,
+                            B.W.p};
         add(sp, sizeof sp);
     }
     const void* bufs[] = {c->table.p, c->ctr.p, c->ulist.p, c->erecs.p, c->plist.p, c->tiles2.p, c->fblist.p,
@@ -884,9 +915,10 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
 
 // Device job of n stacks (results in device memory).  With `graph`, the job
 // is captured once and the graph replayed while the signature repeats.
-int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st) {
+int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st,
+            const StackJob* extra = nullptr) {
     Plan pl;
-    int rc = prepare(c, pl, n, ps, ios);
+    int rc = prepare(c, pl, n, ps, ios, true, extra);
     if (rc) return rc;
     begin_stats(pl, st);
     if (pl.P == 0) {
@@ -1177,7 +1209,7 @@ void clm_destroy(clm_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (StackBufs& B : c->sb) {
-        DevBuf* sbufs[] = {&B.P64, &B.P32, &B.Bop, &B.nrm, &B.astk, &B.recs, &B.ph32, &B.ph64, &B.shiftv, &B.H};
+        DevBuf* sbufs[] = {&B.P64, &B.P32, &B.Bop, &B.nrm, &B.astk, &B.recs, &B.ph32, &B.ph64, &B.shiftv, &B.H, &B.W};
         for (DevBuf* b : sbufs) release(*b);
     }
     for (TileCache& tc : c->tcache) release(tc.tiles);
@@ -1243,8 +1275,9 @@ int clm_clmatrix_grouped(clm_ctx* c, int32_t n, const clm_params* p, const clm_s
     return run_job(c, n, p, io, (flags & CLM_GROUP_GRAPH) != 0, st);
 }
 
-int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, int32_t* cl2, int32_t* jidx,
-                 double* corr, double* shift1d, clm_stats* st) {
+// Host-buffer form of one stack: copy pf in, run, copy the outputs back.
+static int host_job(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, int32_t* cl2, int32_t* jidx,
+                    double* corr, double* shift1d, clm_stats* st, const StackJob* extra) {
     if (!c) return CLM_E_ARG;
     if (!p) return fail(c, CLM_E_ARG, "null params");
     if (p->n_theta % 2) return fail(c, CLM_E_ARG, "n_theta must be even");
@@ -1263,7 +1296,7 @@ int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, 
     if (pf_bytes) HIPCHK(c, hipMemcpyAsync(c->pf.p, pf, pf_bytes, hipMemcpyHostToDevice, c->stream));
     const clm_stack_io io{c->pf.p, (int32_t*)c->out_cl1.p, (int32_t*)c->out_cl2.p, (int32_t*)c->out_j.p,
                           (double*)c->out_corr.p, (double*)c->out_shift.p};
-    rc = run_job(c, 1, p, &io, false, st);
+    rc = run_job(c, 1, p, &io, false, st, extra);
     if (rc) return rc;
     if (P > 0) {
         if (cl1) HIPCHK(c, hipMemcpyAsync(cl1, c->out_cl1.p, P * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1274,6 +1307,42 @@ int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, 
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return CLM_OK;
+}
+
+int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, int32_t* cl2, int32_t* jidx,
+                 double* corr, double* shift1d, clm_stats* st) {
+    return host_job(c, p, pf, cl1, cl2, jidx, corr, shift1d, st, nullptr);
+}
+
+static int check_radius(clm_ctx* c, int32_t r) {
+    // cryo_average_clmap's weights W0 = 1.25.^(21:-1:1) cover 1 + 2d <= 21
+    if (r < 1 || r > 10) return fail(c, CLM_E_ARG, "map_filter_radius %d outside [1, 10]", r);
+    return CLM_OK;
+}
+
+int clm_clmatrix_filtered(clm_ctx* c, const clm_params* p, int32_t map_filter_radius, const void* pf, int32_t* cl1,
+                          int32_t* cl2, int32_t* jidx, double* corr, double* shift1d, clm_stats* st) {
+    if (!c) return CLM_E_ARG;
+    int rc = check_radius(c, map_filter_radius);
+    if (rc) return rc;
+    StackJob ex;
+    ex.filt = map_filter_radius;
+    return host_job(c, p, pf, cl1, cl2, jidx, corr, shift1d, st, &ex);
+}
+
+int clm_clmatrix_filtered_device(clm_ctx* c, const clm_params* p, int32_t map_filter_radius, const void* d_pf,
+                                 int32_t* d_cl1, int32_t* d_cl2, int32_t* d_jidx, double* d_corr, double* d_shift1d,
+                                 clm_stats* st) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    int rc = check_radius(c, map_filter_radius);
+    if (rc) return rc;
+    if (!d_pf && p->K > 1) return fail(c, CLM_E_ARG, "null pf");
+    HIPCHK(c, hipSetDevice(c->device));
+    StackJob ex;
+    ex.filt = map_filter_radius;
+    const clm_stack_io io{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
+    return run_job(c, 1, p, &io, false, st, &ex);
 }
 
 int clm_assemble(int32_t K, const int32_t* cl1, const int32_t* cl2, const double* corr, double* clstack,

@@ -20,7 +20,7 @@ struct DevBuf {
 };
 
 struct StackBufs {
-    DevBuf P64, P32, Bop, nrm, astk, recs, ph32, ph64, shiftv, H;
+    DevBuf P64, P32, Bop, nrm, astk, recs, ph32, ph64, shiftv, H, W;
 };
 
 // Pass-1 tiles of one group, kept while the group's shapes repeat.

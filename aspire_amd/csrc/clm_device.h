@@ -71,6 +71,8 @@ struct StackDev {
     int CPP;                  // waves per pair
     long long pair_base;      // first flat pair index of this stack within its group
     long long npairs;
+    int filt;                 // map_filter_radius d (0: no map filter)
+    const double* wts;        // (2d+1)^2 filter weights, row-major
 };
 
 // Pass-2 / fallback list entry: (stack, slot)

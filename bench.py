@@ -308,6 +308,24 @@ def configs_bench(args):
                                            corr_bit_identical=bool(np.array_equal(g["corr"][same], ora["best"][same])),
                                            uncertain=stats["uncertain_pairs"], refined=stats["refined_pairs"])
             log(f"{cs['name']} {prec}: {row['precisions'][prec]}")
+        if cs["K"] == 100:
+            # argument 6 (map_filter_radius = 1): the filtered FP64 search, every pair
+            t = time.perf_counter()
+            oraf = clm_oracle.pairs(Pr, cs["ms"], cs["step"], k1[sel].astype(np.int32), k2[sel].astype(np.int32),
+                                    nthreads=nth, map_filter_radius=1)
+            cpu_f = time.perf_counter() - t
+            clmatrix_records(pf, cs["ms"], cs["step"], map_filter_radius=1)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            rec, stats = clmatrix_records(pf, cs["ms"], cs["step"], map_filter_radius=1)
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t
+            g = {k: v.cpu().numpy()[sel] for k, v in rec.items()}
+            ident = all(np.array_equal(g[a_], oraf[b_]) for a_, b_ in
+                        (("cl1", "cl1"), ("cl2", "cl2"), ("j", "j"), ("corr", "best"), ("shift", "shift")))
+            row["map_filter_radius_1"] = dict(seconds=sec, pairs_per_s=P / sec, records_bit_identical=ident,
+                                              cpu_oracle_pairs_per_s=sel.size / cpu_f)
+            log(f"{cs['name']} map_filter_radius=1: {row['map_filter_radius_1']}")
         out.append(row)
     print(json.dumps({"mode": "configs", "cases": out}))
 

@@ -135,6 +135,26 @@ int clm_clmatrix_device(clm_ctx *ctx, const clm_params *p, const void *d_pf,
                         double *d_corr, double *d_shift1d, clm_stats *stats);
 
 /*
+ * cryo_clmatrix with map_filter_radius > 0 (its sixth argument,
+ * cryo_clmatrix_cpu.m:24-29,383-385): every shift's L x 2L correlation map
+ * is filtered by cryo_average_clmap (sinograms/cryo_average_clmap.m:20-48,
+ * default weights W0 = 1.25.^(21:-1:1)) before the first-maximum search, and
+ * corr[e] is the filtered value.  Radius d in [1, 10].  Every map entry and
+ * the stencil are evaluated in FP64 with the oracle's arithmetic (the
+ * precision field is not used).  Host-buffer and device-buffer forms, with
+ * the outputs of clm_clmatrix / clm_clmatrix_device.
+ */
+int clm_clmatrix_filtered(clm_ctx *ctx, const clm_params *p,
+                          int32_t map_filter_radius, const void *pf,
+                          int32_t *cl1, int32_t *cl2, int32_t *jidx,
+                          double *corr, double *shift1d, clm_stats *stats);
+int clm_clmatrix_filtered_device(clm_ctx *ctx, const clm_params *p,
+                                 int32_t map_filter_radius, const void *d_pf,
+                                 int32_t *d_cl1, int32_t *d_cl2,
+                                 int32_t *d_jidx, double *d_corr,
+                                 double *d_shift1d, clm_stats *stats);
+
+/*
  * Several independent stacks in one device job (batched cryo_clmatrix calls,
  * e.g. the per-class or per-dataset calls a pipeline makes in a loop around
  * cryo_clmatrix, abinitio/C1/cryo_abinitio_C1_worker.m:99).  Stacks may

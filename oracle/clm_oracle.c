@@ -248,6 +248,103 @@ double clmo_entry(const double *P, int R, int L, double max_shift,
 }
 
 /*
+ * The whole correlation map C = [C1, C2] (L x 2L, column-major: C(l1, l2p) at
+ * map[l2p*L + l1]) of pair (k1, k2) at shift index j, each entry computed as
+ * clmo_entry computes it.
+ */
+void clmo_map(const double *P, int R, int L, double max_shift, double shift_step,
+              int k1, int k2, int j, double *map)
+{
+    for (int l2p = 0; l2p < 2 * L; l2p++)
+        for (int l1 = 0; l1 < L; l1++)
+            map[(int64_t)l2p * L + l1] = clmo_entry(P, R, L, max_shift, shift_step, k1, k2, j, l1, l2p);
+}
+
+/*
+ * cryo_average_clmap(map, d) with its default weights
+ * (sinograms/cryo_average_clmap.m:20-48), as the direct sum its FFT
+ * evaluates:
+ *   W0 = 1.25.^(21:-1:1);  W(i,j) = W0(1 + |i-d| + |j-d|), i,j = 0..2d   (:24-26)
+ *   map_ext = [map; map(:, [L+1:2L, 1:L])]  (2L x 2L)                   (:30)
+ *   out(l1, c) = sum_i sum_j W(i,j) * map_ext((l1-d+i) mod 2L, (c-d+j) mod 2L)
+ * for l1 < L, c < 2L -- the circular cross-correlation of :33-35 moved by d
+ * (:48).  Sums run i outer, j inner.  out is L x 2L column-major.
+ */
+void clmo_filter_map(const double *map, int L, int d, double *out)
+{
+    const int w = 2 * d + 1, L2 = 2 * L;
+    double W[21 * 21];
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < w; j++)
+            W[i * w + j] = pow(1.25, (double)(21 - (abs(i - d) + abs(j - d))));
+    for (int c = 0; c < L2; c++)
+        for (int l1 = 0; l1 < L; l1++) {
+            double v = 0.0;
+            for (int i = 0; i < w; i++) {
+                const int rr = ((l1 - d + i) % L2 + L2) % L2;
+                for (int j = 0; j < w; j++) {
+                    int cc = ((c - d + j) % L2 + L2) % L2;
+                    int r = rr;
+                    if (r >= L) { r -= L; cc = (cc + L) % L2; }  /* rows L..2L-1: halves swapped */
+                    v = v + W[i * w + j] * map[(int64_t)cc * L + r];
+                }
+            }
+            out[(int64_t)c * L + l1] = v;
+        }
+}
+
+/*
+ * The pair loop with map_filter_radius = d > 0 (cryo_clmatrix_cpu.m:383-385):
+ * every shift's map is filtered before the first-max search, and the stored
+ * correlation is the filtered value.  Outputs as clmo_pairs.
+ */
+void clmo_pairs_filtered(const double *P, int R, int L, int K, double max_shift,
+                         double shift_step, int d, const int32_t *k1s, const int32_t *k2s,
+                         int64_t npairs, int32_t *cl1, int32_t *cl2, int32_t *jbest,
+                         double *best, double *shift1d, int nthreads)
+{
+    (void)K;
+    const int S = clmo_n_shifts(max_shift, shift_step);
+    const size_t msz = (size_t)2 * L * L;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        double *map = (double *)malloc(sizeof(double) * msz);
+        double *out = (double *)malloc(sizeof(double) * msz);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int64_t p = 0; p < npairs; p++) {
+            double bv = 0.0, bs = 0.0;
+            int32_t b1 = 0, b2 = 0, bj = -1;
+            for (int j = 0; j < S; j++) {
+                clmo_map(P, R, L, max_shift, shift_step, k1s[p], k2s[p], j, map);
+                clmo_filter_map(map, L, d, out);
+                double sv = NAN;
+                int64_t si = 0;
+                for (int64_t q = 0; q < (int64_t)msz; q++) {
+                    const double v = out[q];
+                    if (isnan(v)) continue;
+                    if (isnan(sv) || v > sv) { sv = v; si = q; }
+                }
+                if (sv > bv) {
+                    bv = sv;
+                    b1 = (int32_t)(si % L) + 1;
+                    b2 = (int32_t)(si / L) + 1;
+                    bj = j;
+                    bs = clmo_shift_value(max_shift, shift_step, j);
+                }
+            }
+            cl1[p] = b1; cl2[p] = b2; jbest[p] = bj; best[p] = bv; shift1d[p] = bs;
+        }
+        free(map);
+        free(out);
+    }
+}
+
+/*
  * Shift-equation rows :722-741.  For each pair in (k1 asc, k2 asc) order,
  * given cl1=clstack(k1,k2), cl2=clstack(k2,k1) (1-based, possibly 0) writes
  * four coefficients and four 1-based column indices.  dtheta = pi/L (:245).

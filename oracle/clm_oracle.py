@@ -53,6 +53,15 @@ def lib():
         L.clmo_equations.restype = None
         L.clmo_equations.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
                                      ctypes.c_int64, _f64p, _i32p]
+        L.clmo_map.restype = None
+        L.clmo_map.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
+        L.clmo_filter_map.restype = None
+        L.clmo_filter_map.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p]
+        L.clmo_pairs_filtered.restype = None
+        L.clmo_pairs_filtered.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, _i32p, _i32p,
+                                          ctypes.c_int64, _i32p, _i32p, _i32p, _f64p, _f64p, ctypes.c_int]
         L.clmo_max_threads.restype = ctypes.c_int
         L.clmo_pft_freqs.restype = None
         L.clmo_pft_freqs.argtypes = [ctypes.c_int, ctypes.c_int, _f64p]
@@ -93,8 +102,10 @@ def all_pairs(K: int):
 
 
 def pairs(P: np.ndarray, max_shift: float, shift_step: float, k1=None, k2=None,
-          nthreads: int = 0):
-    """Run the pair loop (cryo_clmatrix_cpu.m:296-399) on a list of pairs.
+          nthreads: int = 0, map_filter_radius: int = 0):
+    """Run the pair loop (cryo_clmatrix_cpu.m:296-399) on a list of pairs;
+    with ``map_filter_radius`` d > 0 every map is filtered first
+    (cryo_average_clmap, :383-385).
 
     Returns dict with int32 ``cl1``, ``cl2`` (1-based, 0 = not set),
     ``j`` (0-based shift index or -1), float64 ``best`` and ``shift``.
@@ -108,10 +119,33 @@ def pairs(P: np.ndarray, max_shift: float, shift_step: float, k1=None, k2=None,
     out = dict(cl1=np.zeros(n, np.int32), cl2=np.zeros(n, np.int32),
                j=np.zeros(n, np.int32), best=np.zeros(n), shift=np.zeros(n))
     Pf = np.ascontiguousarray(P.astype(np.complex128)).view(np.float64).ravel()
+    if map_filter_radius > 0:
+        lib().clmo_pairs_filtered(Pf, R, L, K, float(max_shift), float(shift_step), int(map_filter_radius),
+                                  k1, k2, n, out["cl1"], out["cl2"], out["j"], out["best"], out["shift"],
+                                  int(nthreads))
+        return out
     lib().clmo_pairs(Pf, R, L, K, float(max_shift), float(shift_step), k1, k2, n,
                      out["cl1"], out["cl2"], out["j"], out["best"], out["shift"],
                      int(nthreads))
     return out
+
+
+def corr_map(P: np.ndarray, max_shift, shift_step, k1, k2, j) -> np.ndarray:
+    """C = [C1, C2] (L x 2L) of one pair at shift index j."""
+    K, L, R = P.shape
+    Pf = np.ascontiguousarray(P.astype(np.complex128)).view(np.float64).ravel()
+    out = np.zeros(2 * L * L)
+    lib().clmo_map(Pf, R, L, float(max_shift), float(shift_step), int(k1), int(k2), int(j), out)
+    return out.reshape(2 * L, L).T
+
+
+def filter_map(m: np.ndarray, d: int) -> np.ndarray:
+    """cryo_average_clmap(m, d) (default weights) of an L x 2L map."""
+    L = m.shape[0]
+    flat = np.ascontiguousarray(np.asarray(m, np.float64).T).ravel()  # column-major
+    out = np.zeros(2 * L * L)
+    lib().clmo_filter_map(flat, L, int(d), out)
+    return out.reshape(2 * L, L).T
 
 
 def entry(P: np.ndarray, max_shift, shift_step, k1, k2, j, l1, l2p) -> float:
