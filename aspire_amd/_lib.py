@@ -30,7 +30,7 @@ EXPORTS = (
     "clm_shiftnet_nparam", "clm_shiftnet_features", "clm_shiftnet_forward",
     "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
     "clm_pft_device", "clm_pft", "clm_pft_freqs", "clm_estimate_shifts",
-    "clm_clmatrix_filtered", "clm_clmatrix_filtered_device",
+    "clm_clmatrix_filtered", "clm_clmatrix_filtered_device", "clm_syncmatrix_vote",
 )
 
 CLM_GROUP_GRAPH = 1
@@ -148,6 +148,17 @@ class ClmLsqrStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ClmVoteStats(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_int64),
+        ("ms_build", ctypes.c_float),
+        ("ms_vote", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -220,6 +231,8 @@ def load():
     L.clm_estimate_shifts.restype = ctypes.c_int
     L.clm_estimate_shifts.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.c_double,
                                       ctypes.c_int32, ctypes.c_double, vp, ctypes.POINTER(ClmLsqrStats)]
+    L.clm_syncmatrix_vote.restype = ctypes.c_int
+    L.clm_syncmatrix_vote.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ClmVoteStats)]
     L.clm_pft_freqs.restype = ctypes.c_int
     L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L

@@ -2,9 +2,10 @@
 
 ``python -m aspire_amd.build`` compiles, in parallel, ``aspire_amd/csrc/
 clm_api.hip`` (the engine: context, orchestration and its kernels),
-``shiftnet.hip`` (the shift network) and ``pair_variants.hip`` once per
-(operand type, shift-grid symmetry), and links them into
-``aspire_amd/libclm.so``.
+``shiftnet.hip`` (the shift network), ``pft.hip`` (the polar Fourier
+transform), ``shifts.hip`` (the 2-D shift least squares), ``vote.hip`` (the
+voting synchronization matrix) and ``pair_variants.hip`` once per (operand
+type, shift-grid symmetry), and links them into ``aspire_amd/libclm.so``.
 """
 from __future__ import annotations
 
@@ -20,7 +21,8 @@ CSRC = os.path.join(PKG, "csrc")
 # (source, extra flags): the engine, the shift network, and the pair-kernel
 # instantiations, one object per (operand type, shift-grid symmetry)
 UNITS = [(os.path.join(CSRC, "clm_api.hip"), []), (os.path.join(CSRC, "shiftnet.hip"), []),
-         (os.path.join(CSRC, "pft.hip"), []), (os.path.join(CSRC, "shifts.hip"), [])] + [
+         (os.path.join(CSRC, "pft.hip"), []), (os.path.join(CSRC, "shifts.hip"), []),
+         (os.path.join(CSRC, "vote.hip"), [])] + [
     (os.path.join(CSRC, "pair_variants.hip"), [f"-DCLM_VARIANT_OP={op}", f"-DCLM_VARIANT_SYM={sym}"])
     for op in ("bf16", "f16", "f32") for sym in (1, 0)]
 ARCH = "gfx950"

@@ -1,0 +1,256 @@
+// vote.hip -- the common-lines voting synchronization matrix on the device:
+// the consumer of cryo_clmatrix's clstack in the C1 ab-initio pipeline.
+//
+// Replaces S = cryo_syncmatrix_vote(clstack, n_theta)
+// (abinitio/C1/sync2n/cryo_syncmatrix_vote.m:38-57, called from
+// abinitio/C1/cryo_abinitio_C1_worker.m:160) without reference rotations.
+// For every pair (k1, k2) with a common line:
+//  * every third image k3 with lines to both gives a triangle; its angle
+//    phi = acos((c3 - c1 c2) / (sin t1 sin t2)) in degrees, t1 = (l13-l12)2pi/L,
+//    t2 = (l21-l23)2pi/L, t3 = (l32-l31)2pi/L, kept when
+//    1 + 2c1c2c3 - (c1^2+c2^2+c3^2) > 1e-5            (cryo_voteIJ.m:40-63)
+//  * the angles vote in a Gaussian histogram (sigma 3) on 60 points of
+//    [0, 180]; the first peak wins; triangles within 6 degrees of it are kept
+//    (cryo_voteIJ.m:90-107)
+//  * each kept triangle gives a relative rotation (rotratio_eulerangle_vec.m);
+//    their top-left 2x2 blocks are averaged into S's block (k1, k2) and its
+//    transpose into (k2, k1) (cryo_syncmatrixIJ_vote.m:125-164).
+// A pair without a common line, or without voted triangles, gets a zero
+// block; the diagonal blocks are the identity.  cos_phi2 and c_alpha are
+// clamped to [-1, 1] before acos.
+//
+// One workgroup per pair (persistent).  Histogram: each wave takes chunks of
+// 64 third images -- one angle per lane -- then each lane owns one of the 60
+// histogram points and adds the chunk's 64 Gaussians in k3 order; the four
+// waves' histograms are added in a fixed order.  Selection and rotations:
+// one thread per third image, fixed-order block sums.  The work is FP64
+// transcendental evaluations (60 exponentials per triangle); memory is the
+// clstack in two 16-bit orientations (row k1/k2 and column k1/k2 reads are
+// contiguous) and S.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "clm_ctx.h"
+
+namespace vote {
+
+constexpr int NT = 60;  // histogram points (ntics)
+
+// M[i*K + j] = clstack(i, j), MT[j*K + i] = clstack(i, j) (1-based lines, 0 = none)
+__global__ void build_kernel(int K, const int* cl1, const int* cl2, unsigned short* M, unsigned short* MT) {
+    const int k1 = blockIdx.x;
+    const long long e0 = (long long)k1 * K - (long long)k1 * (k1 + 1) / 2;
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        if (j <= k1) {
+            if (j == k1) {
+                M[(size_t)k1 * K + k1] = 0;
+                MT[(size_t)k1 * K + k1] = 0;
+            }
+            continue;
+        }
+        const long long e = e0 + (j - k1 - 1);
+        const unsigned short a = (unsigned short)cl1[e], b = (unsigned short)cl2[e];
+        M[(size_t)k1 * K + j] = a;   // clstack(k1, k2)
+        M[(size_t)j * K + k1] = b;   // clstack(k2, k1)
+        MT[(size_t)j * K + k1] = a;
+        MT[(size_t)k1 * K + j] = b;
+    }
+}
+
+__global__ void diag_kernel(int K, double* S) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const size_t n2 = 2 * (size_t)K;
+    S[(2 * (size_t)k) * n2 + 2 * k] = 1.0;
+    S[(2 * (size_t)k + 1) * n2 + 2 * k] = 0.0;
+    S[(2 * (size_t)k) * n2 + 2 * k + 1] = 0.0;
+    S[(2 * (size_t)k + 1) * n2 + 2 * k + 1] = 1.0;
+}
+
+__device__ inline double tick(int d) { return d == NT - 1 ? 180.0 : d * (180.0 / (NT - 1)); }
+
+// Angle (degrees) of triangle (k1, k2, k3), NaN when k3 takes no part.
+__device__ inline double vote_angle(const unsigned short* r1, const unsigned short* r2, const unsigned short* c1_,
+                                    const unsigned short* c2_, int k1, int k3, double l12, double l21, double L) {
+    const double l13 = r1[k3], l23 = r2[k3];
+    if (k3 == k1 || l13 == 0 || l23 == 0) return NAN;
+    const double l31 = c1_[k3], l32 = c2_[k3];
+    const double theta1 = (l13 - l12) * 2 * M_PI / L;
+    const double theta2 = (l21 - l23) * 2 * M_PI / L;
+    const double theta3 = (l32 - l31) * 2 * M_PI / L;
+    const double c1 = cos(theta1), c2 = cos(theta2), c3 = cos(theta3);
+    const double cond = 1 + 2 * c1 * c2 * c3 - (c1 * c1 + c2 * c2 + c3 * c3);
+    if (!(cond > 1.0e-5)) return NAN;
+    double cp = (c3 - c1 * c2) / (sin(theta1) * sin(theta2));
+    cp = cp > 1 ? 1 : (cp < -1 ? -1 : cp);
+    return acos(cp) * 180 / M_PI;
+}
+
+__global__ __launch_bounds__(256) void vote_kernel(int K, int n_theta, const unsigned short* M,
+                                                   const unsigned short* MT, double* S) {
+    __shared__ double s_ang[4][64];
+    __shared__ double s_h[4][64];
+    __shared__ double s_red[4][256];
+    __shared__ int s_cnt[256];
+    __shared__ int s_peak;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long P = (long long)K * (K - 1) / 2;
+    const double L = (double)n_theta;
+    const size_t n2 = 2 * (size_t)K;
+    for (long long e = blockIdx.x; e < P; e += gridDim.x) {
+        // pair (k1, k2) of flat index e (k1 < k2, row-major over k1)
+        int k1 = (int)floor(((2.0 * K - 1.0) - sqrt((2.0 * K - 1.0) * (2.0 * K - 1.0) - 8.0 * (double)e)) / 2.0);
+        if (k1 < 0) k1 = 0;
+        auto first = [&](long long x) { return x * K - x * (x + 1) / 2; };
+        while (k1 > 0 && first(k1) > e) k1--;
+        while (first(k1 + 1) <= e) k1++;
+        const int k2 = (int)(e - first(k1)) + k1 + 1;
+        const unsigned short* r1 = M + (size_t)k1 * K;   // clstack(k1, :)
+        const unsigned short* r2 = M + (size_t)k2 * K;   // clstack(k2, :)
+        const unsigned short* c1 = MT + (size_t)k1 * K;  // clstack(:, k1)
+        const unsigned short* c2 = MT + (size_t)k2 * K;  // clstack(:, k2)
+        const double l12 = r1[k2], l21 = r2[k1];
+        double r00 = 0, r01 = 0, r10 = 0, r11 = 0;
+        if (l12 != 0) {
+            // ---- histogram of the triangle angles
+            double h = 0.0;
+            const double x = lane < NT ? tick(lane) : 0.0;
+            for (int base = 64 * wave; base < K; base += 256) {
+                const int k3 = base + lane;
+                s_ang[wave][lane] = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L) : NAN;
+                __builtin_amdgcn_wave_barrier();
+                for (int q = 0; q < 64; q++) {
+                    const double a = s_ang[wave][q];
+                    if (isnan(a)) continue;  // uniform across the wave
+                    double t = 2 * a * x;
+                    t = t - a * a;
+                    t = t - x * x;
+                    h = h + exp(t / 18.0);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            s_h[wave][lane] = h;
+            __syncthreads();
+            if (tid == 0) {
+                int pk = -1;
+                double best = 0.0;
+                for (int d = 0; d < NT; d++) {
+                    const double v = ((s_h[0][d] + s_h[1][d]) + s_h[2][d]) + s_h[3][d];
+                    if (pk < 0 || v > best) {
+                        best = v;
+                        pk = d;
+                    }
+                }
+                s_peak = best > 0.0 ? pk : -1;  // no triangle voted
+            }
+            __syncthreads();
+            // ---- triangles within 6 degrees of the peak: relative rotations
+            const int pk = s_peak;
+            int cnt = 0;
+            if (pk >= 0) {
+                const double xp = tick(pk);
+                const double aa = (l12 - 1) * 2 * M_PI / L, bb = (l21 - 1) * 2 * M_PI / L;
+                const double ang1 = M_PI - bb, ang3 = aa - M_PI;
+                const double sa = sin(ang1), ca = cos(ang1), sc = sin(ang3), cc = cos(ang3);
+                for (int k3 = tid; k3 < K; k3 += 256) {
+                    const double ang = vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L);
+                    if (isnan(ang) || !(fabs(ang - xp) < 360.0 / NT)) continue;
+                    cnt++;
+                    const double idx1 = (double)c2[k3] - (double)c1[k3];  // cl(k3,k2) - cl(k3,k1)
+                    const double idx2 = (double)r2[k3] - l21;             // cl(k2,k3) - cl(k2,k1)
+                    const double idx3 = (double)r1[k3] - l12;             // cl(k1,k3) - cl(k1,k2)
+                    const double a = cos(2 * M_PI * idx1 / L);
+                    const double b = cos(2 * M_PI * idx2 / L);
+                    const double c = cos(2 * M_PI * idx3 / L);
+                    const double cond = 1 + 2 * a * b * c - (a * a + b * b + c * c);
+                    if (!(cond > 1.0e-5)) continue;
+                    double ca_ = (a - b * c) / sqrt(1 - b * b) / sqrt(1 - c * c);
+                    const double TOL_idx = 1e-12;
+                    const bool ind1 = (idx3 > L / 2 + TOL_idx) || (idx3 < -TOL_idx && idx3 > -L / 2);
+                    const bool ind2 = (idx2 > L / 2 + TOL_idx) || (idx2 < -TOL_idx && idx2 > -L / 2);
+                    if (ind1 != ind2) ca_ = -ca_;
+                    ca_ = ca_ > 1 ? 1 : (ca_ < -1 ? -1 : ca_);
+                    const double cb = cos(acos(ca_));
+                    r00 = r00 + (cc * ca - sc * cb * sa);
+                    r01 = r01 + (-cc * sa - sc * cb * ca);
+                    r10 = r10 + (sc * ca + cc * cb * sa);
+                    r11 = r11 + (-sa * sc + cc * cb * ca);
+                }
+            }
+            s_red[0][tid] = r00;
+            s_red[1][tid] = r01;
+            s_red[2][tid] = r10;
+            s_red[3][tid] = r11;
+            s_cnt[tid] = cnt;
+            __syncthreads();
+            for (int s = 128; s > 0; s >>= 1) {
+                if (tid < s) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) s_red[m][tid] += s_red[m][tid + s];
+                    s_cnt[tid] += s_cnt[tid + s];
+                }
+                __syncthreads();
+            }
+            const int n = s_cnt[0];
+            if (n > 0) {
+                r00 = s_red[0][0] / n;
+                r01 = s_red[1][0] / n;
+                r10 = s_red[2][0] / n;
+                r11 = s_red[3][0] / n;
+            } else {
+                r00 = r01 = r10 = r11 = 0.0;
+            }
+            __syncthreads();
+        }
+        if (tid < 4) {
+            const int a = tid >> 1, b = tid & 1;
+            const double v = tid == 0 ? r00 : tid == 1 ? r01 : tid == 2 ? r10 : r11;
+            S[(2 * (size_t)k2 + b) * n2 + 2 * (size_t)k1 + a] = v;  // S(2k1+a, 2k2+b) = R22(a,b)
+            S[(2 * (size_t)k1 + a) * n2 + 2 * (size_t)k2 + b] = v;  // S(2k2+b, 2k1+a)
+        }
+    }
+}
+
+}  // namespace vote
+
+using namespace clmh;
+
+extern "C" int clm_syncmatrix_vote(clm_ctx* c, int32_t K, int32_t n_theta, const int32_t* d_cl1,
+                                   const int32_t* d_cl2, double* d_S, clm_vote_stats* st) {
+    using namespace vote;
+    if (!c) return CLM_E_ARG;
+    if (st) *st = clm_vote_stats{};
+    if (K < 1 || K > 65536) return fail(c, CLM_E_ARG, "K = %d outside [1, 65536]", K);
+    if (n_theta <= 0 || n_theta > 65535) return fail(c, CLM_E_ARG, "invalid n_theta %d", n_theta);
+    if (!d_S || (K > 1 && (!d_cl1 || !d_cl2))) return fail(c, CLM_E_ARG, "null pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    const size_t kk = (size_t)K * K;
+    if ((rc = ensure(c, c->vt_m, kk * 2)) || (rc = ensure(c, c->vt_mt, kk * 2))) return rc;
+    unsigned short* M = (unsigned short*)c->vt_m.p;
+    unsigned short* MT = (unsigned short*)c->vt_mt.p;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    build_kernel<<<K, 256, 0, c->stream>>>(K, d_cl1, d_cl2, M, MT);
+    diag_kernel<<<(K + 255) / 256, 256, 0, c->stream>>>(K, d_S);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    const long long P = (long long)K * (K - 1) / 2;
+    if (P > 0) {
+        const long long nb = std::min<long long>(P, (long long)c->n_cu * 8);
+        vote_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, d_S);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        st->ms_build = a;
+        st->ms_vote = b;
+        st->pairs = P;
+    }
+    return CLM_OK;
+}

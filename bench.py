@@ -50,13 +50,14 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (heuristic variant)")
     ap.add_argument("--tune-pairs", type=int, default=200000)
     ap.add_argument("--flags", type=int, default=0, help="clm_params.reserved bits (1: no lagged schedule)")
-    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped", "configs", "pft", "shifts"],
+    ap.add_argument("--mode", default="bench", choices=["bench", "pairs", "copy", "grouped", "configs", "pft", "shifts", "vote"],
                     help="pairs: one BF16 call on the workload and nothing else (for counter "
                          "runs); copy: a plain 1 GiB device copy (HBM byte-counter calibration); "
                          "grouped: the 64-stack uneven batch in one graph-captured grouped job; "
                          "configs: the K=100 and K=1000 workloads in every precision against the oracle; "
                          "pft: the polar Fourier transform of the K=5000 workload's images; "
-                         "shifts: the LSQR solve of the K=5000 workload's shift equations")
+                         "shifts: the LSQR solve of the K=5000 workload's shift equations; "
+                         "vote: the voting synchronization matrix of the K=5000 workload's common lines")
     ap.add_argument("--stacks", type=int, default=64, help="grouped mode: number of stacks")
     ap.add_argument("--grouped-sample", type=int, default=40, help="grouped mode: oracle pairs per stack")
     args = ap.parse_args()
@@ -75,6 +76,8 @@ def main():
         return pft_bench(args)
     if args.mode == "shifts":
         return shifts_bench(args)
+    if args.mode == "vote":
+        return vote_bench(args)
 
     if args.mode == "copy":
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -554,6 +557,69 @@ def shifts_bench(args):
                                       f"scaled to {it} iterations"},
            "parity_sample": {"iterations": n_cpu,
                              "max_abs_diff_rel_to_max": float(np.abs(xg - xo).max() / np.abs(xo).max())}}
+    print(json.dumps(out))
+
+
+def vote_bench(args):
+    """S = cryo_syncmatrix_vote(clstack, 360) of the K=5000 workload's common
+    lines (BF16 engine), on the device; the oracle on a seeded sample of pairs
+    (one thread) as the CPU baseline and parity sample.  The work is FP64
+    transcendental evaluation: the kernel evaluates 60 Gaussians per voting
+    triangle (the exponential count per second is reported)."""
+    import torch
+
+    from aspire_amd import sim
+    from aspire_amd.clmatrix import clmatrix_records
+    from aspire_amd.vote import syncmatrix_vote_records
+    from oracle import clm_oracle
+
+    K, T = args.K, 360
+    st0 = sim.make_stack(K=K, n=129, n_r=65, n_theta=T, snr=1 / 16, max_shift_2d=5.0, shift_kind="uniform",
+                         seed=1003, device="cuda")
+    rec, _ = clmatrix_records(st0.pf, 10, 0.5, precision="bf16")
+    del st0
+    torch.cuda.synchronize()
+    P = K * (K - 1) // 2
+    S = None
+    for i in range(args.warmup):
+        S, st = syncmatrix_vote_records(rec["cl1"], rec["cl2"], K, T)
+        del S
+    times = []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        S, st = syncmatrix_vote_records(rec["cl1"], rec["cl2"], K, T)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"vote step {i}: {times[-1]:.3f}s {st}")
+        if i + 1 < args.steps:
+            del S
+    sec = float(np.mean(times))
+    cl1, cl2 = rec["cl1"].cpu().numpy(), rec["cl2"].cpu().numpy()
+    k1, k2 = np.triu_indices(K, 1)
+    clstack = np.zeros((K, K))
+    clstack[k1, k2] = cl1
+    clstack[k2, k1] = cl2
+    flat = np.ascontiguousarray(clstack.T).ravel()
+    ntri = int(np.count_nonzero(clstack)) // 2  # pairs with a line
+    sample = np.random.default_rng(0).choice(P, size=60, replace=False)
+    t = time.perf_counter()
+    same = 0
+    for e in sample:
+        a, b = int(k1[e]), int(k2[e])
+        ref = clm_oracle.vote_pair(clstack, T, a, b, _flat=flat)
+        same += int(np.abs(S[2 * a:2 * a + 2, 2 * b:2 * b + 2].cpu().numpy() - ref).max() <= 1e-10)
+    cpu_sec = time.perf_counter() - t
+    out = {"metric": "pairs/s: cryo_syncmatrix_vote of the K=5000 workload's common lines", "value": P / sec,
+           "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "voting synchronization matrix, K=5000, n_theta=360", "K": K, "pairs": P},
+           "stats": st, "pairs_with_line": ntri,
+           "exp_per_s_upper_bound": 60.0 * P * (K - 2) / sec,
+           "roofline": None,
+           "cpu_baseline": {"value": sample.size / cpu_sec, "unit": "pairs/s", "cores": 1, "kind": "port",
+                            "sample": f"{sample.size} seeded pairs, FP64 C oracle (one thread)"},
+           "parity_sample": {"pairs": int(sample.size), "blocks_equal_1e-10": same}}
     print(json.dumps(out))
 
 

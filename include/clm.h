@@ -370,6 +370,28 @@ int clm_estimate_shifts(clm_ctx *ctx, int32_t K, int32_t n_theta,
                         const double *d_shift1d, double tol, int32_t maxit,
                         double damp, double *d_x, clm_lsqr_stats *stats);
 
+/*
+ * The voting synchronization matrix on the device: the consumer of the
+ * common lines in the C1 ab-initio pipeline.  Replaces
+ *   S = cryo_syncmatrix_vote(clstack, n_theta)
+ * (abinitio/C1/sync2n/cryo_syncmatrix_vote.m:38-57 with
+ * cryo_syncmatrixIJ_vote.m, cryo_voteIJ.m and rotratio_eulerangle_vec.m,
+ * no reference rotations), reading the common lines from the per-pair
+ * outputs of clm_clmatrix_device (d_cl1, d_cl2).  d_S (device, 2K x 2K
+ * doubles, column-major) receives S: identity diagonal blocks, the voted
+ * 2x2 block of every pair at (k1, k2) and its transpose at (k2, k1).
+ * cos_phi2 and c_alpha are clamped to [-1, 1] before acos.
+ */
+typedef struct clm_vote_stats {
+    int64_t pairs;  /* K(K-1)/2                           */
+    float ms_build; /* clstack tables and diagonal blocks */
+    float ms_vote;  /* the vote                           */
+} clm_vote_stats;
+
+int clm_syncmatrix_vote(clm_ctx *ctx, int32_t K, int32_t n_theta,
+                        const int32_t *d_cl1, const int32_t *d_cl2,
+                        double *d_S, clm_vote_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,10 @@ def lib():
         L.clmo_pairs_filtered.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, _i32p, _i32p,
                                           ctypes.c_int64, _i32p, _i32p, _i32p, _f64p, _f64p, ctypes.c_int]
+        L.clmo_vote_pair.restype = None
+        L.clmo_vote_pair.argtypes = [_f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p]
+        L.clmo_syncmatrix_vote.restype = None
+        L.clmo_syncmatrix_vote.argtypes = [_f64p, ctypes.c_int, ctypes.c_double, _f64p, ctypes.c_int]
         L.clmo_max_threads.restype = ctypes.c_int
         L.clmo_pft_freqs.restype = None
         L.clmo_pft_freqs.argtypes = [ctypes.c_int, ctypes.c_int, _f64p]
@@ -223,3 +227,25 @@ def cryo_pft(p: np.ndarray, n_r: int, n_theta: int, nthreads: int = 0) -> np.nda
     lib().clmo_pft(flat, n, K, n_r, n_theta, out, nthreads)
     pf = out.view(np.complex128).reshape(K, n_theta, n_r)
     return pf.transpose(2, 1, 0)
+
+
+# ------------------------------------------------------ synchronization by voting
+def _cl_colmajor(clstack):
+    cl = np.asarray(clstack, dtype=np.float64)
+    return np.ascontiguousarray(cl.T).ravel()  # column-major K x K
+
+
+def syncmatrix_vote(clstack, L, nthreads: int = 0) -> np.ndarray:
+    """S (2K x 2K) of cryo_syncmatrix_vote(clstack, L) (vote_oracle.c)."""
+    K = np.shape(clstack)[0]
+    S = np.zeros(4 * K * K)
+    lib().clmo_syncmatrix_vote(_cl_colmajor(clstack), int(K), float(L), S, int(nthreads))
+    return S.reshape(2 * K, 2 * K).T
+
+
+def vote_pair(clstack, L, k1, k2, _flat=None) -> np.ndarray:
+    """The 2x2 block of pair (k1, k2) (0-based)."""
+    K = np.shape(clstack)[0]
+    r = np.zeros(4)
+    lib().clmo_vote_pair(_cl_colmajor(clstack) if _flat is None else _flat, int(K), float(L), int(k1), int(k2), r)
+    return r.reshape(2, 2)
