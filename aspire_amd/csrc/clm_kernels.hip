@@ -235,8 +235,12 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             stk = a.ulist[item].stack + a.stack_shift;
             slot = a.ulist[item].slot;
         }
-        const StackDev& sd = a.stacks[stk];
-        const Geom& g = sd.g;
+        // the stack's geometry and pointers in registers: results are stored
+        // through sd, so references into it would be re-read on every row
+        const StackDev& sd = a.stacks[__builtin_amdgcn_readfirstlane(stk)];
+        const Geom g = sd.g;
+        const float2* const P32 = sd.P32;
+        const float2* const ph32 = sd.ph32;
         const int kq = g.Rp / 4;  // multiple of 4
         const PairRec& rec = sd.recs[slot];
         const int k1 = rec.k1, k2 = rec.k2;
@@ -274,9 +278,10 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             if (row >= 0) {
                 e = row / g.L;
                 l1 = row - e * g.L;
-                const float4* p = (const float4*)(sd.P32 + ((size_t)k1 * g.L + l1) * g.Rp + q * kq);
-                const float4* bb = (const float4*)(sd.P32 + ((size_t)k2 * g.L + col) * g.Rp + q * kq);
-                const float4* ph = (const float4*)(sd.ph32 + (size_t)e * g.Rp + q * kq);
+                const float4* p = (const float4*)(P32 + ((size_t)k1 * g.L + l1) * g.Rp + q * kq);
+                const float4* bb = (const float4*)(P32 + ((size_t)k2 * g.L + col) * g.Rp + q * kq);
+                const float4* ph = (const float4*)(ph32 + (size_t)e * g.Rp + q * kq);
+#pragma unroll 4
                 for (int i = 0; i < kq / 2; i++) {
                     const float4 pv = p[i], qv = ph[i], b4 = bb[i];
                     const float ar0 = fmaf(pv.x, qv.x, -pv.y * qv.y);

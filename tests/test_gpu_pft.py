@@ -66,6 +66,20 @@ def test_pft_matches_oracle(oracle, n, n_r, n_theta, K, dt):
     assert np.array_equal(host, got)
 
 
+def test_pft_image_chunks(oracle):
+    # more than 8192 images: the stack is staged in chunks
+    say("case pft-chunks")
+    rng = np.random.default_rng(77)
+    n, n_r, n_theta, K = 9, 5, 8, 8200
+    p = rng.standard_normal((n, n, K))
+    st = {}
+    pf, _ = gpft.cryo_pft(to_dev(p), n_r, n_theta, stats=st)
+    assert st["chunks"] == 2
+    got = pf.cpu().numpy()
+    ref = oracle.cryo_pft(p, n_r, n_theta)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_pft_empty_stack():
     import torch
 
