@@ -44,8 +44,8 @@ __device__ inline float wave_max(float v) {
 // CPR 16-byte chunks (re half, then im half).  Chunk c of row r is stored at
 // position (c + r) mod CPR, so the MFMA fragment reads (32 rows at the same
 // logical chunk) hit distinct LDS banks without row padding.  Rows past
-// the last shift are zero.  An A-tile is one contiguous block that the pair
-// kernel copies to LDS with direct global-to-LDS loads.
+// the last shift are zero.  An A-tile is one contiguous block in this layout,
+// so the pair kernel copies it to LDS chunk for chunk.
 template <typename OpT, int NK> struct ATile {
     static constexpr int Rp = NK * Op<OpT>::KSTEP;
     static constexpr int CPR = 2 * Rp * (int)sizeof(OpT) / 16;  // chunks per row
@@ -206,8 +206,12 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         // private memory
         typedef int v4i __attribute__((ext_vector_type(4)));
         v4i pre[NPT];
+        // through a global-address-space pointer: flat loads would also
+        // count against the LDS wait counter, so every wait for an A
+        // fragment would wait for this prefetch as well
+        typedef __attribute__((address_space(1))) const v4i gv4i;
         auto load_pre = [&](int at) {
-            const v4i* src = (const v4i*)(asrc + (size_t)at * NCH);
+            gv4i* src = (gv4i*)(asrc + (size_t)at * NCH);
 #pragma unroll
             for (int j = 0; j < NPT; j++) {
                 const int q = tid + j * (int)blockDim.x;
@@ -325,6 +329,25 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         // the other buffer and start loading tile at+2; reduce (lagging
         // waves: the previous tile), multiply, reduce (leading waves: this
         // tile)
+        // A fragments of k-step ks (real and imaginary halves of the row)
+        auto frag_re = [&](const int4* buf, int ks) -> frag {
+            const char* arow = (const char*)buf + lc * CPR * 16;
+            if constexpr (O::KSTEP == 16) {
+                return *(const frag*)(arow + AT::phys(lc, 2 * ks + h) * 16);
+            } else {
+                const int kk = h * (Rp / 2) + ks;
+                return *(const float*)(arow + AT::phys(lc, kk / 4) * 16 + (kk % 4) * 4);
+            }
+        };
+        auto frag_im = [&](const int4* buf, int ks) -> frag {
+            const char* arow = (const char*)buf + lc * CPR * 16;
+            if constexpr (O::KSTEP == 16) {
+                return *(const frag*)(arow + AT::phys(lc, CPR / 2 + 2 * ks + h) * 16);
+            } else {
+                const int kk = h * (Rp / 2) + ks;
+                return *(const float*)(arow + AT::phys(lc, CPR / 2 + kk / 4) * 16 + (kk % 4) * 4);
+            }
+        };
         auto step = [&](int at, const int4* cur, int4* nxt) {
             if (at + 1 < g.NAT) {
                 store_pre(nxt);
@@ -337,28 +360,42 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
             }
             if (a.prio) __builtin_amdgcn_s_setprio(1);
             if (active) {
-                const char* arow = (const char*)cur + lc * CPR * 16;
+                // fragments of k-steps 0 and 1 first; each fragment register
+                // is refilled two k-steps ahead as soon as the MFMAs that
+                // read it have issued, so a refill has CT + 2 CT MFMAs to
+                // arrive (the schedule is pinned below)
+                frag fa[2], fb[2];
+                fa[0] = frag_re(cur, 0);
+                fb[0] = frag_im(cur, 0);
+                if (NK > 1) {
+                    fa[1] = frag_re(cur, 1);
+                    fb[1] = frag_im(cur, 1);
+                }
 #pragma unroll
                 for (int ks = 0; ks < NK; ks++) {
-                    frag a0, b0;
-                    if constexpr (O::KSTEP == 16) {
-                        a0 = *(const frag*)(arow + AT::phys(lc, 2 * ks + h) * 16);
-                        b0 = *(const frag*)(arow + AT::phys(lc, CPR / 2 + 2 * ks + h) * 16);
-                    } else {
-                        const int kk = h * (Rp / 2) + ks;
-                        a0 = *(const float*)(arow + AT::phys(lc, kk / 4) * 16 + (kk % 4) * 4);
-                        b0 = *(const float*)(arow + AT::phys(lc, CPR / 2 + kk / 4) * 16 + (kk % 4) * 4);
+                    const int s = ks & 1;
+#pragma unroll
+                    for (int ct = 0; ct < CT; ct++) {
+                        f32x16 z = {0.f};
+                        X[ct] = O::mma(fa[s], Bre[ct][ks], ks == 0 ? z : X[ct]);
                     }
 #pragma unroll
                     for (int ct = 0; ct < CT; ct++) {
-                        if (ks == 0) {
-                            f32x16 z = {0.f};
-                            X[ct] = O::mma(a0, Bre[ct][0], z);
-                            Y[ct] = O::mma(b0, Bim[ct][0], z);
-                        } else {
-                            X[ct] = O::mma(a0, Bre[ct][ks], X[ct]);
-                            Y[ct] = O::mma(b0, Bim[ct][ks], Y[ct]);
-                        }
+                        f32x16 z = {0.f};
+                        Y[ct] = O::mma(fb[s], Bim[ct][ks], ks == 0 ? z : Y[ct]);
+                    }
+                    if (ks + 2 < NK) {
+                        fa[s] = frag_re(cur, ks + 2);
+                        fb[s] = frag_im(cur, ks + 2);
+                    }
+                }
+                if constexpr (O::KSTEP == 16) {
+                    // scheduling groups (0x100: LDS reads, 0x008: MFMAs)
+                    __builtin_amdgcn_sched_group_barrier(0x100, NK > 1 ? 4 : 2, 0);
+#pragma unroll
+                    for (int ks = 0; ks < NK; ks++) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 2 * CT, 0);
+                        if (ks + 2 < NK) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
                     }
                 }
             }
