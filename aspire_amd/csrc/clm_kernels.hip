@@ -215,7 +215,8 @@ __device__ inline void write_result(const StackDev& sd, int slot, double v, long
 constexpr int RS_MAXV = 2 * MAXE;  // FP32 values kept per pair (2 per row)
 
 // One wave per pair.  FP32 re-score with 4 lanes per candidate row (16 rows
-// per step, contiguous 16-byte loads), values kept in LDS; then FP64 re-score
+// per step, the row's lanes reading adjacent 16-byte pieces), values kept in
+// LDS; then FP64 re-score
 // of every candidate within tau of the FP32 best, one lane per candidate with
 // the oracle's sequential arithmetic.
 __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
@@ -226,6 +227,9 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
     const int q = lane & 3, rl = lane >> 2;  // k-quarter, row within the step
     const bool listed = a.mode != 0, emit = a.mode == 1;
     const long long n_items = listed ? (long long)min(*a.n_items_dev, a.item_cap) : a.n_items;
+    // statistics, summed in lane 0 over the wave's items and added once at
+    // the end: an atomic per pair on one address serialises across the chip
+    unsigned long long c_cell = 0, c_emit = 0, c_64 = 0;
     for (long long item = (long long)blockIdx.x * W + wave; item < n_items; item += (long long)gridDim.x * W) {
         int stk, slot;
         if (!listed) {
@@ -257,14 +261,14 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
                 continue;
             }
             nrows = 16 * rec.ncells;
-            if (lane == 0 && a.ncell) atomicAdd(a.ncell, (unsigned long long)rec.ncells);
+            c_cell += (unsigned long long)rec.ncells;
         } else {
             if (er->overflow) {
                 if (lane == 0) { const int u = atomicAdd(a.fb_count, 1); a.fb_list[u] = SlotRef{stk, slot}; }
                 continue;
             }
             nrows = er->n;
-            if (lane == 0 && a.nemit) atomicAdd(a.nemit, (unsigned long long)nrows);
+            c_emit += (unsigned long long)nrows;
         }
         // ---- FP32 re-score
         float bv = -INFINITY;
@@ -278,12 +282,15 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             if (row >= 0) {
                 e = row / g.L;
                 l1 = row - e * g.L;
-                const float4* p = (const float4*)(P32 + ((size_t)k1 * g.L + l1) * g.Rp + q * kq);
-                const float4* bb = (const float4*)(P32 + ((size_t)k2 * g.L + col) * g.Rp + q * kq);
-                const float4* ph = (const float4*)(ph32 + (size_t)e * g.Rp + q * kq);
+                // the 4 lanes of a row read 4 adjacent 16-byte pieces per
+                // load (lane q: pieces q, q + 4, ...), so one load touches
+                // 16 half cache lines instead of 64 lines
+                const float4* p = (const float4*)(P32 + ((size_t)k1 * g.L + l1) * g.Rp) + q;
+                const float4* bb = (const float4*)(P32 + ((size_t)k2 * g.L + col) * g.Rp) + q;
+                const float4* ph = (const float4*)(ph32 + (size_t)e * g.Rp) + q;
 #pragma unroll 4
                 for (int i = 0; i < kq / 2; i++) {
-                    const float4 pv = p[i], qv = ph[i], b4 = bb[i];
+                    const float4 pv = p[4 * i], qv = ph[4 * i], b4 = bb[4 * i];
                     const float ar0 = fmaf(pv.x, qv.x, -pv.y * qv.y);
                     const float ai0 = fmaf(pv.x, qv.y, pv.y * qv.x);
                     const float ar1 = fmaf(pv.z, qv.z, -pv.w * qv.w);
@@ -355,12 +362,17 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
                 if (better64(ov, ok, b64, k64)) { b64 = ov; k64 = ok; }
                 n64 += __shfl_xor(n64, o, 64);
             }
-            if (lane == 0 && a.n64) atomicAdd(a.n64, (unsigned long long)n64);
+            c_64 += (unsigned long long)n64;
             fv = b64;
             fk = k64;
         }
         if (lane == 0) write_result(sd, slot, fv, fk);
         __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) {
+        if (a.ncell && c_cell) atomicAdd(a.ncell, c_cell);
+        if (a.nemit && c_emit) atomicAdd(a.nemit, c_emit);
+        if (a.n64 && c_64) atomicAdd(a.n64, c_64);
     }
 }
 
@@ -601,32 +613,43 @@ __global__ void window_merge_kernel(int K, int L, const double* __restrict__ lsh
                                     uint8_t* __restrict__ flag, unsigned long long* __restrict__ counts) {
     const long long P = (long long)K * (K - 1) / 2;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= P) return;
-    int k1, k2;
-    slot_to_pair(e, K, k1, k2);
-    const int c1 = wcl1[e], c2 = wcl2[e];
-    const double v = wcorr[e];
-    int ok = 0, j = -1;
-    double sfull = 0.0;
-    if (c1 > 0 && v >= gate) {
-        const double a1 = lsh[(size_t)k1 * L + (c1 - 1)];
-        const int l2 = (c2 - 1) % L;
-        const double a2 = lsh[(size_t)k2 * L + l2];
-        sfull = c2 <= L ? wshift[e] + a1 - a2 : wshift[e] - a1 - a2;
-        j = (int)rint((sfull + max_shift) / step);
-        ok = j >= 0 && j < S_full;
+    const bool valid = e < P;
+    bool ok = false, gated = false;
+    if (valid) {
+        int k1, k2;
+        slot_to_pair(e, K, k1, k2);
+        const int c1 = wcl1[e], c2 = wcl2[e];
+        const double v = wcorr[e];
+        int j = -1;
+        gated = c1 > 0 && v >= gate;
+        if (gated) {
+            const double a1 = lsh[(size_t)k1 * L + (c1 - 1)];
+            const int l2 = (c2 - 1) % L;
+            const double a2 = lsh[(size_t)k2 * L + l2];
+            const double sfull = c2 <= L ? wshift[e] + a1 - a2 : wshift[e] - a1 - a2;
+            j = (int)rint((sfull + max_shift) / step);
+            ok = j >= 0 && j < S_full;
+        }
+        if (ok) {
+            if (cl1) cl1[e] = c1;
+            if (cl2) cl2[e] = c2;
+            if (jidx) jidx[e] = j;
+            if (corr) corr[e] = v;
+            if (shift1d) shift1d[e] = shiftv_full[j];
+            flag[e] = 0;
+        } else {
+            flag[e] = 1;
+        }
     }
-    if (ok) {
-        if (cl1) cl1[e] = c1;
-        if (cl2) cl2[e] = c2;
-        if (jidx) jidx[e] = j;
-        if (corr) corr[e] = v;
-        if (shift1d) shift1d[e] = shiftv_full[j];
-        flag[e] = 0;
-        atomicAdd(counts + 0, 1ull);
-    } else {
-        flag[e] = 1;
-        atomicAdd(counts + (c1 > 0 && v >= gate ? 2 : 1), 1ull);  // 1: below gate, 2: off the grid
+    // counts 0: accepted, 1: below the gate, 2: off the grid; one atomic per
+    // wave and kind (an atomic per pair on one address serialises)
+    const unsigned long long m0 = __ballot(valid && ok);
+    const unsigned long long m1 = __ballot(valid && !gated);
+    const unsigned long long m2 = __ballot(valid && gated && !ok);
+    if ((threadIdx.x & 63) == 0) {
+        if (m0) atomicAdd(counts + 0, (unsigned long long)__popcll(m0));
+        if (m1) atomicAdd(counts + 1, (unsigned long long)__popcll(m1));
+        if (m2) atomicAdd(counts + 2, (unsigned long long)__popcll(m2));
     }
 }
 
