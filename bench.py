@@ -534,6 +534,14 @@ def shifts_bench(args):
     it = st["iterations"]
     dev_ms = float(np.mean([s_["ms"] for s_ in sts]))
     achieved = 32.0 * P * it / (dev_ms * 1e-3)
+    # measured HBM bytes of one solve (rocprof FETCH_SIZE/WRITE_SIZE, calibrated),
+    # valid for the workload and iteration count it was measured on
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "lsqr_k5000_traffic.json")
+    if os.path.exists(tpath) and K == 5000:
+        tj = json.load(open(tpath))
+        if tj.get("iterations") == it:
+            traffic = tj["hbm_bytes_per_launch"]
     # CPU oracle: same system, a few iterations
     cl1, cl2, b = (rec[k].cpu().numpy() for k in ("cl1", "cl2", "shift"))
     A, bb = shifts_oracle.system(K, n_theta, cl1, cl2, b)
@@ -550,7 +558,7 @@ def shifts_bench(args):
                       "tol": 1e-8, "maxit": 100},
            "lsqr": st,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
-                        "frac": achieved / 8e12, "traffic": None, "kernel": "whole solve",
+                        "frac": achieved / 8e12, "traffic": traffic, "kernel": "whole solve",
                         "bytes_per_pair_iteration": 32},
            "cpu_baseline": {"value": P / (cpu_it * it), "unit": "pairs/s", "cores": 1, "kind": "port",
                             "sample": f"{n_cpu} LSQR iterations of the oracle (scipy CSR) on the full system, "
