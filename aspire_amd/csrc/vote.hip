@@ -20,8 +20,9 @@
 // clamped to [-1, 1] before acos.
 //
 // One workgroup per pair (persistent).  Histogram: each wave takes chunks of
-// 64 third images -- one angle per lane -- then each lane owns one of the 60
-// histogram points and adds the chunk's 64 Gaussians in k3 order; the four
+// 64 third images -- one angle per lane, passed to every lane through scalar
+// registers -- then each lane owns one of the 60 histogram points and adds
+// the chunk's 64 Gaussians in k3 order; the four
 // waves' histograms are added in a fixed order.  Selection and rotations:
 // one thread per third image, fixed-order block sums.  The work is FP64
 // transcendental evaluations (60 exponentials per triangle); memory is the
@@ -90,7 +91,6 @@ __device__ inline double vote_angle(const unsigned short* r1, const unsigned sho
 
 __global__ __launch_bounds__(256) void vote_kernel(int K, int n_theta, const unsigned short* M,
                                                    const unsigned short* MT, double* S) {
-    __shared__ double s_ang[4][64];
     __shared__ double s_h[4][64];
     __shared__ double s_red[4][256];
     __shared__ int s_cnt[256];
@@ -119,17 +119,20 @@ __global__ __launch_bounds__(256) void vote_kernel(int K, int n_theta, const uns
             const double x = lane < NT ? tick(lane) : 0.0;
             for (int base = 64 * wave; base < K; base += 256) {
                 const int k3 = base + lane;
-                s_ang[wave][lane] = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L) : NAN;
-                __builtin_amdgcn_wave_barrier();
+                const double ang = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L) : NAN;
+                // lane q's angle to every lane through scalar registers
+                const long long ab = __double_as_longlong(ang);
+                const int alo = (int)ab, ahi = (int)(ab >> 32);
                 for (int q = 0; q < 64; q++) {
-                    const double a = s_ang[wave][q];
+                    const unsigned lo = (unsigned)__builtin_amdgcn_readlane(alo, q);
+                    const long long hi = (long long)__builtin_amdgcn_readlane(ahi, q);
+                    const double a = __longlong_as_double((hi << 32) | lo);
                     if (isnan(a)) continue;  // uniform across the wave
                     double t = 2 * a * x;
                     t = t - a * a;
                     t = t - x * x;
                     h = h + exp(t / 18.0);
                 }
-                __builtin_amdgcn_wave_barrier();
             }
             s_h[wave][lane] = h;
             __syncthreads();
