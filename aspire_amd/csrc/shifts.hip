@@ -31,7 +31,9 @@ namespace lsq {
 struct State {
     double alpha, beta, bnorm, phibar, rhobar, anorm, res2, rnorm, arnorm;
     double damp, tol;
+    double t1, t2, ia;  // this iteration's update of x, w and v
     int itn, maxit, done, flag;
+    int upd;            // 1: this iteration's scalar step ran, apply its update
 };
 
 __device__ inline long long first(long long k1, long long K) { return k1 * K - k1 * (k1 + 1) / 2; }
@@ -262,27 +264,43 @@ __global__ __launch_bounds__(1024) void vinit_kernel(int n2, const double* vt_ro
     }
 }
 
-// One LSQR step after the new u and beta: v <- A'u - beta v, alpha, the
-// damping and plane rotations, x and w, the stopping tests.
-__global__ __launch_bounds__(1024) void step_kernel(int n2, const double* vt_row, const double* vt_col, State* S,
-                                                    double* v, double* w, double* x) {
+// One LSQR step after the new u and beta, in three launches:
+//  stepa (one thread per element): v <- A'u / beta - beta v, and per-block
+//    partial sums of its squares (fixed order);
+//  stepb (one workgroup): alpha = |v|, the damping and plane rotations, the
+//    stopping tests; the update's coefficients go to the state;
+//  stepc (one thread per element): v <- v / alpha, x and w.
+// beta = 0 (b in the range of A): v and alpha stay, the rotation below makes
+// the last update of x and phibar becomes 0.  stepc applies the update of
+// the iteration whose tests pass, as the last one.
+__global__ __launch_bounds__(256) void stepa_kernel(int n2, const double* vt_row, const double* vt_col,
+                                                    const State* S, double* v, double* part) {
+    __shared__ double sh[256];
+    if (S->done || !(S->beta > 0.0)) return;
+    const double beta = S->beta, ib = 1.0 / beta;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < n2) {
+        t = atu_value(n2, vt_row, vt_col, i) * ib - beta * v[i];
+        v[i] = t;
+    }
+    const double sq = block_sum<256>(t * t, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = sq;
+}
+
+__global__ __launch_bounds__(1024) void stepb_kernel(int nb, const double* part, State* S) {
     __shared__ double sh[1024];
-    __shared__ double coef[2];
-    if (S->done) return;
-    // beta = 0 (b in the range of A): v and alpha stay, the rotation below
-    // makes the last update of x and phibar becomes 0
+    if (S->done) {
+        if (threadIdx.x == 0) S->upd = 0;
+        return;
+    }
     const double beta = S->beta;
     double alpha = S->alpha;
     if (beta > 0.0) {
-        const double ib = 1.0 / beta;
-        double s = 0.0;
-        for (int i = threadIdx.x; i < n2; i += 1024) {
-            const double t = atu_value(n2, vt_row, vt_col, i) * ib - beta * v[i];
-            v[i] = t;
-            s += t * t;
-        }
-        s = block_sum<1024>(s, sh);
-        alpha = sqrt(s);
+        double sq = 0.0;
+        for (int i = threadIdx.x; i < nb; i += 1024) sq += part[i];
+        sq = block_sum<1024>(sq, sh);
+        alpha = sqrt(sq);
     }
     if (threadIdx.x == 0) {
         const double dampsq = S->damp * S->damp;
@@ -297,8 +315,10 @@ __global__ __launch_bounds__(1024) void step_kernel(int n2, const double* vt_row
         const double phi = cs * phibar1;
         S->phibar = sn * phibar1;
         const double tau = sn * phi;
-        coef[0] = phi / rho;
-        coef[1] = -theta / rho;
+        S->t1 = phi / rho;
+        S->t2 = -theta / rho;
+        S->ia = beta > 0.0 ? (alpha > 0.0 ? 1.0 / alpha : 0.0) : 1.0;
+        S->upd = 1;
         S->alpha = alpha;
         S->res2 += psi * psi;
         S->rnorm = sqrt(S->phibar * S->phibar + S->res2);
@@ -312,10 +332,13 @@ __global__ __launch_bounds__(1024) void step_kernel(int n2, const double* vt_row
             S->flag = 1;
         }
     }
-    __syncthreads();
-    const double t1 = coef[0], t2 = coef[1];
-    const double ia = beta > 0.0 ? (alpha > 0.0 ? 1.0 / alpha : 0.0) : 1.0;
-    for (int i = threadIdx.x; i < n2; i += 1024) {
+}
+
+__global__ __launch_bounds__(256) void stepc_kernel(int n2, const State* S, double* v, double* w, double* x) {
+    if (!S->upd) return;
+    const double t1 = S->t1, t2 = S->t2, ia = S->ia;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n2) {
         const double vi = v[i] * ia;
         v[i] = vi;
         const double wi = w[i];
@@ -367,6 +390,7 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     init_kernel<<<K, 256, 0, c->stream>>>(K, L, d_cl1, d_cl2, d_shift1d, pack, u, part);
     beta_kernel<<<1, 1024, 0, c->stream>>>(K, part, S, 1);
     const int natu = K + (K + 15) / 16 * NS;
+    const int nstep = (n2 + 255) / 256;  // <= K: its partial sums fit in part
     atu_kernel<<<natu, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row, vt_col);
     vinit_kernel<<<1, 1024, 0, c->stream>>>(n2, vt_row, vt_col, S, v, w, d_x);
     HIPCHK(c, hipGetLastError());
@@ -376,7 +400,9 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
         av_kernel<<<K, 256, 0, c->stream>>>(K, pack, ta, tb, v, S, u, part);
         beta_kernel<<<1, 1024, 0, c->stream>>>(K, part, S, 0);
         atu_kernel<<<natu, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row, vt_col);
-        step_kernel<<<1, 1024, 0, c->stream>>>(n2, vt_row, vt_col, S, v, w, d_x);
+        stepa_kernel<<<nstep, 256, 0, c->stream>>>(n2, vt_row, vt_col, S, v, part);
+        stepb_kernel<<<1, 1024, 0, c->stream>>>(nstep, part, S);
+        stepc_kernel<<<nstep, 256, 0, c->stream>>>(n2, S, v, w, d_x);
         if (it % 8 == 7 && it + 1 < maxit) {
             int done = 0;
             HIPCHK(c, hipGetLastError());
