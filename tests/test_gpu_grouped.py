@@ -112,6 +112,24 @@ def test_grouped_graph_replays(stacks):
     assert not np.array_equal(host(fresh[0])["cl1"], eager[0]["cl1"])
 
 
+def test_grouped_statistics_add_up(stacks):
+    """The re-score's counters are summed per wave and added once per wave.
+    A pair's re-scored cells are those within the certified margin of its
+    final maximum, whichever kernel variant or job it runs in, so one
+    grouped job must report the sums of the single-stack calls."""
+    say("case grouped-statistics")
+    precs = ["bf16", "fp16", "bf16", "bf16"]
+    dev = [to_dev(pf) for pf in stacks]
+    _, st = clmatrix_grouped(dev, MS, SS, precs, graph=False)
+    sts = [clmatrix_records(d, MS[i], SS[i], precision=precs[i])[1] for i, d in enumerate(dev)]
+    say(st)
+    for k in ("pairs", "cells_rescored", "fp64_candidates", "uncertain_pairs", "refined_pairs"):
+        assert st[k] == sum(x[k] for x in sts), (k, st[k], [x[k] for x in sts])
+    # every certain pair re-scores at least the cell and the FP64 candidate of its maximum
+    certain = st["pairs"] - st["uncertain_pairs"]
+    assert st["cells_rescored"] >= certain and st["fp64_candidates"] >= certain
+
+
 def test_grouped_empty_and_tiny():
     import torch
 
