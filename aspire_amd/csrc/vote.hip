@@ -89,101 +89,126 @@ __device__ inline double vote_angle(const unsigned short* r1, const unsigned sho
     return acos(cp) * 180 / M_PI;
 }
 
-// three waves per SIMD (<= 170 VGPRs): the histogram loop is a chain of
-// dependent FP64 operations per angle, so occupancy hides its latency
-// (measured 4.46 s against 5.03 s at two waves per SIMD, K = 5000)
-__global__ __launch_bounds__(256, 3) void vote_kernel(int K, int n_theta, const unsigned short* M,
-                                                   const unsigned short* MT, double* S) {
+// Pair (k1, k2) of flat index e (k1 < k2, row-major over k1).
+__device__ inline void pair_of(long long e, int K, int& k1, int& k2) {
+    int a = (int)floor(((2.0 * K - 1.0) - sqrt((2.0 * K - 1.0) * (2.0 * K - 1.0) - 8.0 * (double)e)) / 2.0);
+    if (a < 0) a = 0;
+    auto first = [&](long long x) { return x * K - x * (x + 1) / 2; };
+    while (a > 0 && first(a) > e) a--;
+    while (first(a + 1) <= e) a++;
+    k1 = a;
+    k2 = (int)(e - first(a)) + a + 1;
+}
+
+// Histogram of the triangle angles and its first peak, one workgroup per
+// pair; peak[e] = the peak's point, -1 when the pair has no common line or
+// no triangle voted.  A kernel of its own, so that its few registers allow
+// many waves per SIMD: the loop is a chain of dependent FP64 operations per
+// angle, and occupancy hides its latency.
+__global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const unsigned short* M,
+                                                   const unsigned short* MT, int* peak) {
     __shared__ double s_h[4][64];
-    __shared__ double s_red[4][256];
-    __shared__ int s_cnt[256];
-    __shared__ int s_peak;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const long long P = (long long)K * (K - 1) / 2;
     const double L = (double)n_theta;
-    const size_t n2 = 2 * (size_t)K;
     for (long long e = blockIdx.x; e < P; e += gridDim.x) {
-        // pair (k1, k2) of flat index e (k1 < k2, row-major over k1)
-        int k1 = (int)floor(((2.0 * K - 1.0) - sqrt((2.0 * K - 1.0) * (2.0 * K - 1.0) - 8.0 * (double)e)) / 2.0);
-        if (k1 < 0) k1 = 0;
-        auto first = [&](long long x) { return x * K - x * (x + 1) / 2; };
-        while (k1 > 0 && first(k1) > e) k1--;
-        while (first(k1 + 1) <= e) k1++;
-        const int k2 = (int)(e - first(k1)) + k1 + 1;
+        int k1, k2;
+        pair_of(e, K, k1, k2);
         const unsigned short* r1 = M + (size_t)k1 * K;   // clstack(k1, :)
         const unsigned short* r2 = M + (size_t)k2 * K;   // clstack(k2, :)
         const unsigned short* c1 = MT + (size_t)k1 * K;  // clstack(:, k1)
         const unsigned short* c2 = MT + (size_t)k2 * K;  // clstack(:, k2)
         const double l12 = r1[k2], l21 = r2[k1];
+        if (l12 == 0) {
+            if (tid == 0) peak[e] = -1;
+            continue;
+        }
+        double h = 0.0;
+        const double x = lane < NT ? tick(lane) : 0.0;
+        for (int base = 64 * wave; base < K; base += 256) {
+            const int k3 = base + lane;
+            const double ang = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L) : NAN;
+            // lane q's angle to every lane through scalar registers
+            const long long ab = __double_as_longlong(ang);
+            const int alo = (int)ab, ahi = (int)(ab >> 32);
+            for (int q = 0; q < 64; q++) {
+                const unsigned lo = (unsigned)__builtin_amdgcn_readlane(alo, q);
+                const long long hi = (long long)__builtin_amdgcn_readlane(ahi, q);
+                const double a = __longlong_as_double((hi << 32) | lo);
+                if (isnan(a)) continue;  // uniform across the wave
+                double t = 2 * a * x;
+                t = t - a * a;
+                t = t - x * x;
+                h = h + exp(t / 18.0);
+            }
+        }
+        s_h[wave][lane] = h;
+        __syncthreads();
+        if (tid == 0) {
+            int pk = -1;
+            double best = 0.0;
+            for (int d = 0; d < NT; d++) {
+                const double v = ((s_h[0][d] + s_h[1][d]) + s_h[2][d]) + s_h[3][d];
+                if (pk < 0 || v > best) {
+                    best = v;
+                    pk = d;
+                }
+            }
+            peak[e] = best > 0.0 ? pk : -1;  // no triangle voted
+        }
+        __syncthreads();
+    }
+}
+
+// Triangles within 6 degrees of each pair's peak: their relative rotations,
+// averaged into the pair's 2x2 block of S and its transpose.
+__global__ __launch_bounds__(256) void rot_kernel(int K, int n_theta, const unsigned short* M,
+                                                  const unsigned short* MT, const int* peak, double* S) {
+    __shared__ double s_red[4][256];
+    __shared__ int s_cnt[256];
+    const int tid = threadIdx.x;
+    const long long P = (long long)K * (K - 1) / 2;
+    const double L = (double)n_theta;
+    const size_t n2 = 2 * (size_t)K;
+    for (long long e = blockIdx.x; e < P; e += gridDim.x) {
+        int k1, k2;
+        pair_of(e, K, k1, k2);
+        const unsigned short* r1 = M + (size_t)k1 * K;
+        const unsigned short* r2 = M + (size_t)k2 * K;
+        const unsigned short* c1 = MT + (size_t)k1 * K;
+        const unsigned short* c2 = MT + (size_t)k2 * K;
+        const double l12 = r1[k2], l21 = r2[k1];
+        const int pk = peak[e];  // -1: no common line or no vote
         double r00 = 0, r01 = 0, r10 = 0, r11 = 0;
-        if (l12 != 0) {
-            // ---- histogram of the triangle angles
-            double h = 0.0;
-            const double x = lane < NT ? tick(lane) : 0.0;
-            for (int base = 64 * wave; base < K; base += 256) {
-                const int k3 = base + lane;
-                const double ang = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L) : NAN;
-                // lane q's angle to every lane through scalar registers
-                const long long ab = __double_as_longlong(ang);
-                const int alo = (int)ab, ahi = (int)(ab >> 32);
-                for (int q = 0; q < 64; q++) {
-                    const unsigned lo = (unsigned)__builtin_amdgcn_readlane(alo, q);
-                    const long long hi = (long long)__builtin_amdgcn_readlane(ahi, q);
-                    const double a = __longlong_as_double((hi << 32) | lo);
-                    if (isnan(a)) continue;  // uniform across the wave
-                    double t = 2 * a * x;
-                    t = t - a * a;
-                    t = t - x * x;
-                    h = h + exp(t / 18.0);
-                }
-            }
-            s_h[wave][lane] = h;
-            __syncthreads();
-            if (tid == 0) {
-                int pk = -1;
-                double best = 0.0;
-                for (int d = 0; d < NT; d++) {
-                    const double v = ((s_h[0][d] + s_h[1][d]) + s_h[2][d]) + s_h[3][d];
-                    if (pk < 0 || v > best) {
-                        best = v;
-                        pk = d;
-                    }
-                }
-                s_peak = best > 0.0 ? pk : -1;  // no triangle voted
-            }
-            __syncthreads();
-            // ---- triangles within 6 degrees of the peak: relative rotations
-            const int pk = s_peak;
+        if (pk >= 0) {
             int cnt = 0;
-            if (pk >= 0) {
-                const double xp = tick(pk);
-                const double aa = (l12 - 1) * 2 * M_PI / L, bb = (l21 - 1) * 2 * M_PI / L;
-                const double ang1 = M_PI - bb, ang3 = aa - M_PI;
-                const double sa = sin(ang1), ca = cos(ang1), sc = sin(ang3), cc = cos(ang3);
-                for (int k3 = tid; k3 < K; k3 += 256) {
-                    const double ang = vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L);
-                    if (isnan(ang) || !(fabs(ang - xp) < 360.0 / NT)) continue;
-                    cnt++;
-                    const double idx1 = (double)c2[k3] - (double)c1[k3];  // cl(k3,k2) - cl(k3,k1)
-                    const double idx2 = (double)r2[k3] - l21;             // cl(k2,k3) - cl(k2,k1)
-                    const double idx3 = (double)r1[k3] - l12;             // cl(k1,k3) - cl(k1,k2)
-                    const double a = cos(2 * M_PI * idx1 / L);
-                    const double b = cos(2 * M_PI * idx2 / L);
-                    const double c = cos(2 * M_PI * idx3 / L);
-                    const double cond = 1 + 2 * a * b * c - (a * a + b * b + c * c);
-                    if (!(cond > 1.0e-5)) continue;
-                    double ca_ = (a - b * c) / sqrt(1 - b * b) / sqrt(1 - c * c);
-                    const double TOL_idx = 1e-12;
-                    const bool ind1 = (idx3 > L / 2 + TOL_idx) || (idx3 < -TOL_idx && idx3 > -L / 2);
-                    const bool ind2 = (idx2 > L / 2 + TOL_idx) || (idx2 < -TOL_idx && idx2 > -L / 2);
-                    if (ind1 != ind2) ca_ = -ca_;
-                    ca_ = ca_ > 1 ? 1 : (ca_ < -1 ? -1 : ca_);
-                    const double cb = cos(acos(ca_));
-                    r00 = r00 + (cc * ca - sc * cb * sa);
-                    r01 = r01 + (-cc * sa - sc * cb * ca);
-                    r10 = r10 + (sc * ca + cc * cb * sa);
-                    r11 = r11 + (-sa * sc + cc * cb * ca);
-                }
+            const double xp = tick(pk);
+            const double aa = (l12 - 1) * 2 * M_PI / L, bb = (l21 - 1) * 2 * M_PI / L;
+            const double ang1 = M_PI - bb, ang3 = aa - M_PI;
+            const double sa = sin(ang1), ca = cos(ang1), sc = sin(ang3), cc = cos(ang3);
+            for (int k3 = tid; k3 < K; k3 += 256) {
+                const double ang = vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L);
+                if (isnan(ang) || !(fabs(ang - xp) < 360.0 / NT)) continue;
+                cnt++;
+                const double idx1 = (double)c2[k3] - (double)c1[k3];  // cl(k3,k2) - cl(k3,k1)
+                const double idx2 = (double)r2[k3] - l21;             // cl(k2,k3) - cl(k2,k1)
+                const double idx3 = (double)r1[k3] - l12;             // cl(k1,k3) - cl(k1,k2)
+                const double a = cos(2 * M_PI * idx1 / L);
+                const double b = cos(2 * M_PI * idx2 / L);
+                const double c = cos(2 * M_PI * idx3 / L);
+                const double cond = 1 + 2 * a * b * c - (a * a + b * b + c * c);
+                if (!(cond > 1.0e-5)) continue;
+                double ca_ = (a - b * c) / sqrt(1 - b * b) / sqrt(1 - c * c);
+                const double TOL_idx = 1e-12;
+                const bool ind1 = (idx3 > L / 2 + TOL_idx) || (idx3 < -TOL_idx && idx3 > -L / 2);
+                const bool ind2 = (idx2 > L / 2 + TOL_idx) || (idx2 < -TOL_idx && idx2 > -L / 2);
+                if (ind1 != ind2) ca_ = -ca_;
+                ca_ = ca_ > 1 ? 1 : (ca_ < -1 ? -1 : ca_);
+                const double cb = cos(acos(ca_));
+                r00 = r00 + (cc * ca - sc * cb * sa);
+                r01 = r01 + (-cc * sa - sc * cb * ca);
+                r10 = r10 + (sc * ca + cc * cb * sa);
+                r11 = r11 + (-sa * sc + cc * cb * ca);
             }
             s_red[0][tid] = r00;
             s_red[1][tid] = r01;
@@ -234,18 +259,22 @@ extern "C" int clm_syncmatrix_vote(clm_ctx* c, int32_t K, int32_t n_theta, const
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
     const size_t kk = (size_t)K * K;
-    if ((rc = ensure(c, c->vt_m, kk * 2)) || (rc = ensure(c, c->vt_mt, kk * 2))) return rc;
+    const long long P = (long long)K * (K - 1) / 2;
+    if ((rc = ensure(c, c->vt_m, kk * 2)) || (rc = ensure(c, c->vt_mt, kk * 2)) ||
+        (rc = ensure(c, c->vt_pk, (size_t)std::max<long long>(P, 1) * 4)))
+        return rc;
     unsigned short* M = (unsigned short*)c->vt_m.p;
     unsigned short* MT = (unsigned short*)c->vt_mt.p;
+    int* pk = (int*)c->vt_pk.p;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     build_kernel<<<K, 256, 0, c->stream>>>(K, d_cl1, d_cl2, M, MT);
     diag_kernel<<<(K + 255) / 256, 256, 0, c->stream>>>(K, d_S);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    const long long P = (long long)K * (K - 1) / 2;
     if (P > 0) {
-        const long long nb = std::min<long long>(P, (long long)c->n_cu * 8);
-        vote_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, d_S);
+        const long long nb = std::min<long long>(P, (long long)c->n_cu * 16);
+        peak_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, pk);
+        rot_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, pk, d_S);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
