@@ -1225,7 +1225,7 @@ void clm_destroy(clm_ctx* c) {
     for (DevBuf* b : wb) release(*b);
     DevBuf* pb[] = {&c->pft_ax, &c->pft_ay, &c->pft_xt, &c->pft_img, &c->pft_out,
                     &c->ls_pack, &c->ls_u, &c->ls_part, &c->ls_vec, &c->ls_tab, &c->ls_state,
-                    &c->vt_m, &c->vt_mt, &c->vt_pk};
+                    &c->vt_m, &c->vt_mt, &c->vt_pk, &c->vt_cs};
     for (DevBuf* b : pb) release(*b);
     DevBuf* bufs[] = {&c->table, &c->ctr, &c->ulist, &c->erecs, &c->plist, &c->tiles2, &c->fblist, &c->pf,
                       &c->out_cl1, &c->out_cl2, &c->out_j, &c->out_corr, &c->out_shift};

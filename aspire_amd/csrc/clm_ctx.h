@@ -56,8 +56,9 @@ struct clm_ctx {
     clmh::DevBuf pft_ax, pft_ay, pft_xt, pft_img, pft_out;
     // shift least squares: packed lines, u, row partial sums, 2K vectors, tables, scalar state
     clmh::DevBuf ls_pack, ls_u, ls_part, ls_vec, ls_tab, ls_state;
-    // voting: clstack as 16-bit tables, row-major and transposed; per-pair histogram peaks
-    clmh::DevBuf vt_m, vt_mt, vt_pk;
+    // voting: clstack as 16-bit tables, row-major and transposed; per-pair histogram
+    // peaks; cos and sin of the line-difference angles
+    clmh::DevBuf vt_m, vt_mt, vt_pk, vt_cs;
     // autotuned variants: shape key -> (ct, W, lag)
     std::vector<std::pair<unsigned long long, int4>> tuned;
     clmh::DevBuf tune_tiles;

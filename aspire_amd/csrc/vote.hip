@@ -72,19 +72,38 @@ __global__ void diag_kernel(int K, double* S) {
 
 __device__ inline double tick(int d) { return d == NT - 1 ? 180.0 : d * (180.0 / (NT - 1)); }
 
+// cs[d + n_theta] = (cos, sin) of d * 2 * pi / n_theta for the integer line
+// differences d in [-n_theta, n_theta]: the same expression, evaluated
+// once, as the angle of a line difference in the triangle and rotation
+// formulas (2 * pi * d / L rounds the same way, the factor 2 being exact).
+__global__ void trig_kernel(int n_theta, double2* cs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > 2 * n_theta) return;
+    const double d = (double)(i - n_theta);
+    const double th = d * 2 * M_PI / (double)n_theta;
+    cs[i] = make_double2(cos(th), sin(th));
+}
+
+// Entry of line difference d (clamped to the table, so that a malformed
+// clstack reads in bounds).
+__device__ inline double2 trig(const double2* cs, int nt, int d) {
+    const int i = d + nt;
+    return cs[i < 0 ? 0 : (i > 2 * nt ? 2 * nt : i)];
+}
+
 // Angle (degrees) of triangle (k1, k2, k3), NaN when k3 takes no part.
 __device__ inline double vote_angle(const unsigned short* r1, const unsigned short* r2, const unsigned short* c1_,
-                                    const unsigned short* c2_, int k1, int k3, double l12, double l21, double L) {
-    const double l13 = r1[k3], l23 = r2[k3];
+                                    const unsigned short* c2_, int k1, int k3, int l12, int l21,
+                                    const double2* cs, int nt) {
+    const int l13 = r1[k3], l23 = r2[k3];
     if (k3 == k1 || l13 == 0 || l23 == 0) return NAN;
-    const double l31 = c1_[k3], l32 = c2_[k3];
-    const double theta1 = (l13 - l12) * 2 * M_PI / L;
-    const double theta2 = (l21 - l23) * 2 * M_PI / L;
-    const double theta3 = (l32 - l31) * 2 * M_PI / L;
-    const double c1 = cos(theta1), c2 = cos(theta2), c3 = cos(theta3);
+    const int l31 = c1_[k3], l32 = c2_[k3];
+    // theta1 = (l13 - l12) 2pi/L, theta2 = (l21 - l23) 2pi/L, theta3 = (l32 - l31) 2pi/L
+    const double2 t1 = trig(cs, nt, l13 - l12), t2 = trig(cs, nt, l21 - l23), t3 = trig(cs, nt, l32 - l31);
+    const double c1 = t1.x, c2 = t2.x, c3 = t3.x;
     const double cond = 1 + 2 * c1 * c2 * c3 - (c1 * c1 + c2 * c2 + c3 * c3);
     if (!(cond > 1.0e-5)) return NAN;
-    double cp = (c3 - c1 * c2) / (sin(theta1) * sin(theta2));
+    double cp = (c3 - c1 * c2) / (t1.y * t2.y);
     cp = cp > 1 ? 1 : (cp < -1 ? -1 : cp);
     return acos(cp) * 180 / M_PI;
 }
@@ -106,11 +125,10 @@ __device__ inline void pair_of(long long e, int K, int& k1, int& k2) {
 // many waves per SIMD: the loop is a chain of dependent FP64 operations per
 // angle, and occupancy hides its latency.
 __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const unsigned short* M,
-                                                   const unsigned short* MT, int* peak) {
+                                                   const unsigned short* MT, const double2* cs, int* peak) {
     __shared__ double s_h[4][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const long long P = (long long)K * (K - 1) / 2;
-    const double L = (double)n_theta;
     for (long long e = blockIdx.x; e < P; e += gridDim.x) {
         int k1, k2;
         pair_of(e, K, k1, k2);
@@ -118,7 +136,7 @@ __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const uns
         const unsigned short* r2 = M + (size_t)k2 * K;   // clstack(k2, :)
         const unsigned short* c1 = MT + (size_t)k1 * K;  // clstack(:, k1)
         const unsigned short* c2 = MT + (size_t)k2 * K;  // clstack(:, k2)
-        const double l12 = r1[k2], l21 = r2[k1];
+        const int l12 = r1[k2], l21 = r2[k1];
         if (l12 == 0) {
             if (tid == 0) peak[e] = -1;
             continue;
@@ -127,7 +145,7 @@ __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const uns
         const double x = lane < NT ? tick(lane) : 0.0;
         for (int base = 64 * wave; base < K; base += 256) {
             const int k3 = base + lane;
-            const double ang = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L) : NAN;
+            const double ang = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, cs, n_theta) : NAN;
             // lane q's angle to every lane through scalar registers
             const long long ab = __double_as_longlong(ang);
             const int alo = (int)ab, ahi = (int)(ab >> 32);
@@ -163,7 +181,8 @@ __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const uns
 // Triangles within 6 degrees of each pair's peak: their relative rotations,
 // averaged into the pair's 2x2 block of S and its transpose.
 __global__ __launch_bounds__(256) void rot_kernel(int K, int n_theta, const unsigned short* M,
-                                                  const unsigned short* MT, const int* peak, double* S) {
+                                                  const unsigned short* MT, const double2* cs, const int* peak,
+                                                  double* S) {
     __shared__ double s_red[4][256];
     __shared__ int s_cnt[256];
     const int tid = threadIdx.x;
@@ -177,7 +196,8 @@ __global__ __launch_bounds__(256) void rot_kernel(int K, int n_theta, const unsi
         const unsigned short* r2 = M + (size_t)k2 * K;
         const unsigned short* c1 = MT + (size_t)k1 * K;
         const unsigned short* c2 = MT + (size_t)k2 * K;
-        const double l12 = r1[k2], l21 = r2[k1];
+        const int il12 = r1[k2], il21 = r2[k1];
+        const double l12 = il12, l21 = il21;
         const int pk = peak[e];  // -1: no common line or no vote
         double r00 = 0, r01 = 0, r10 = 0, r11 = 0;
         if (pk >= 0) {
@@ -187,15 +207,17 @@ __global__ __launch_bounds__(256) void rot_kernel(int K, int n_theta, const unsi
             const double ang1 = M_PI - bb, ang3 = aa - M_PI;
             const double sa = sin(ang1), ca = cos(ang1), sc = sin(ang3), cc = cos(ang3);
             for (int k3 = tid; k3 < K; k3 += 256) {
-                const double ang = vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, L);
+                const double ang = vote_angle(r1, r2, c1, c2, k1, k3, il12, il21, cs, n_theta);
                 if (isnan(ang) || !(fabs(ang - xp) < 360.0 / NT)) continue;
                 cnt++;
-                const double idx1 = (double)c2[k3] - (double)c1[k3];  // cl(k3,k2) - cl(k3,k1)
-                const double idx2 = (double)r2[k3] - l21;             // cl(k2,k3) - cl(k2,k1)
-                const double idx3 = (double)r1[k3] - l12;             // cl(k1,k3) - cl(k1,k2)
-                const double a = cos(2 * M_PI * idx1 / L);
-                const double b = cos(2 * M_PI * idx2 / L);
-                const double c = cos(2 * M_PI * idx3 / L);
+                const int d1 = (int)c2[k3] - (int)c1[k3];  // cl(k3,k2) - cl(k3,k1)
+                const int d2 = (int)r2[k3] - il21;         // cl(k2,k3) - cl(k2,k1)
+                const int d3 = (int)r1[k3] - il12;         // cl(k1,k3) - cl(k1,k2)
+                const double idx2 = d2, idx3 = d3;
+                // cos(2 pi idx / L) of the three differences
+                const double a = trig(cs, n_theta, d1).x;
+                const double b = trig(cs, n_theta, d2).x;
+                const double c = trig(cs, n_theta, d3).x;
                 const double cond = 1 + 2 * a * b * c - (a * a + b * b + c * c);
                 if (!(cond > 1.0e-5)) continue;
                 double ca_ = (a - b * c) / sqrt(1 - b * b) / sqrt(1 - c * c);
@@ -261,20 +283,23 @@ extern "C" int clm_syncmatrix_vote(clm_ctx* c, int32_t K, int32_t n_theta, const
     const size_t kk = (size_t)K * K;
     const long long P = (long long)K * (K - 1) / 2;
     if ((rc = ensure(c, c->vt_m, kk * 2)) || (rc = ensure(c, c->vt_mt, kk * 2)) ||
-        (rc = ensure(c, c->vt_pk, (size_t)std::max<long long>(P, 1) * 4)))
+        (rc = ensure(c, c->vt_pk, (size_t)std::max<long long>(P, 1) * 4)) ||
+        (rc = ensure(c, c->vt_cs, (size_t)(2 * n_theta + 1) * sizeof(double2))))
         return rc;
     unsigned short* M = (unsigned short*)c->vt_m.p;
     unsigned short* MT = (unsigned short*)c->vt_mt.p;
     int* pk = (int*)c->vt_pk.p;
+    double2* cs = (double2*)c->vt_cs.p;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     build_kernel<<<K, 256, 0, c->stream>>>(K, d_cl1, d_cl2, M, MT);
+    trig_kernel<<<(2 * n_theta + 1 + 255) / 256, 256, 0, c->stream>>>(n_theta, cs);
     diag_kernel<<<(K + 255) / 256, 256, 0, c->stream>>>(K, d_S);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (P > 0) {
         const long long nb = std::min<long long>(P, (long long)c->n_cu * 16);
-        peak_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, pk);
-        rot_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, pk, d_S);
+        peak_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, cs, pk);
+        rot_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, cs, pk, d_S);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
