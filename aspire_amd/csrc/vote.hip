@@ -19,15 +19,17 @@
 // block; the diagonal blocks are the identity.  cos_phi2 and c_alpha are
 // clamped to [-1, 1] before acos.
 //
-// One workgroup per pair (persistent).  Histogram: each wave takes chunks of
-// 64 third images -- one angle per lane, passed to every lane through scalar
-// registers -- then each lane owns one of the 60 histogram points and adds
-// the chunk's 64 Gaussians in k3 order; the four
-// waves' histograms are added in a fixed order.  Selection and rotations:
-// one thread per third image, fixed-order block sums.  The work is FP64
-// transcendental evaluations (60 exponentials per triangle); memory is the
-// clstack in two 16-bit orientations (row k1/k2 and column k1/k2 reads are
-// contiguous) and S.
+// Two persistent kernels, one workgroup per pair.  peak_kernel: each wave
+// takes chunks of 64 third images -- one angle per lane, passed to every lane
+// through scalar registers -- then each lane owns one of the 60 histogram
+// points and adds the chunk's 64 Gaussians in k3 order; the four waves'
+// histograms are added in a fixed order and the first peak is stored per
+// pair.  rot_kernel: selection and rotations, one thread per third image,
+// fixed-order block sums.  The cosines and sines of the integer line
+// differences come from a table built with the formulas' own expression.
+// The work is FP64 transcendental evaluations (60 exponentials per
+// triangle); memory is the clstack in two 16-bit orientations (row k1/k2
+// and column k1/k2 reads are contiguous), the per-pair peaks and S.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
