@@ -8,9 +8,10 @@ Inputs are resident in HBM when the timed region starts; outputs (per-pair
 records) stay in HBM.
 
 Reports BF16-MFMA + FP32/FP64 re-score pairs/s (value), the plain FP32 GPU
-path's pairs/s on the first 1000 images of the same stack, clstack agreement
-with the FP64 oracle on a seeded sample of pairs, the roofline position of
-the pair kernel, and the oracle's CPU throughput on a bounded sample.
+path's pairs/s on the same K=5000 stack, clstack agreement with the FP64
+oracle on a seeded sample of 20,000 pairs (whose oracle time is also the CPU
+baseline), the roofline position of the pair kernel, and the drop-in's
+end-to-end time from host buffers to the five cryo_clmatrix outputs.
 """
 from __future__ import annotations
 
@@ -40,8 +41,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--K", type=int, default=5000)
     ap.add_argument("--precision", default="bf16")
-    ap.add_argument("--fp32-K", type=int, default=1000)
-    ap.add_argument("--sample-pairs", type=int, default=400)
+    ap.add_argument("--fp32-K", type=int, default=5000)
+    ap.add_argument("--sample-pairs", type=int, default=20000)
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in line (host pf -> the five cryo_clmatrix outputs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
@@ -145,7 +148,9 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    # plain FP32 GPU path on the first fp32-K images of the same stack
+    # plain FP32 GPU path on the same stack (the first fp32-K images; all by
+    # default).  The autotuner's timing launches warm its kernels; the one
+    # timed call includes its own buffer growth, a few ms against ~50 s.
     Kf = min(args.fp32_K, K)
     pf_f = pf[:, :, :Kf]
     Pf = Kf * (Kf - 1) // 2
@@ -153,24 +158,24 @@ def main():
     if not args.no_tune:
         rows32, best32 = autotune(pf_f, ms, step, "fp32", sample_pairs=min(args.tune_pairs, Pf), flags=args.flags)
         tune32 = {"chosen": best32, "variants": rows32}
-    _ = run("fp32", pf_f)
+    else:
+        _ = run("fp32", pf[:, :, :min(Kf, 200)])
     torch.cuda.synchronize()
     t = time.perf_counter()
-    _, st32 = run("fp32", pf_f)
+    rec32, st32 = run("fp32", pf_f)
     torch.cuda.synchronize()
     fp32_sec = time.perf_counter() - t
     fp32_value = Pf / fp32_sec
     log(f"fp32 K={Kf}: {fp32_sec:.3f}s {st32}")
+    del rec32
 
     # clstack agreement with the FP64 oracle on a seeded sample of pairs
     from oracle import clm_oracle
 
-    rng = np.random.default_rng(42)
+    from aspire_amd.workloads import sample_pairs
+
     ns = min(args.sample_pairs, P)
-    slots = np.sort(rng.choice(P, size=ns, replace=False))
-    firsts = np.array([k * K - k * (k + 1) // 2 for k in range(K)], dtype=np.int64)
-    k1 = np.searchsorted(firsts, slots, side="right") - 1
-    k2 = slots - firsts[k1] + k1 + 1
+    slots, k1, k2 = sample_pairs(K, ns, seed=42)
     imgs = np.unique(np.concatenate([k1, k2]))
     idx = {int(v): i for i, v in enumerate(imgs)}
     sub = pf[:, :, torch.as_tensor(imgs, device=pf.device)].cpu().numpy()
@@ -216,8 +221,12 @@ def main():
         clm_oracle.pairs(Psub, ms, step, kk1[:n1], kk2[:n1], nthreads=1)
         one_sec = time.perf_counter() - t
         cpu = {"value": ns / cpu_sec, "unit": "pairs/s", "cores": int(nth), "kind": "port",
-               "sample": f"{ns} seeded pairs of the K=5000 workload, FP64 C oracle (OpenMP)",
-               "value_1thread": n1 / one_sec, "sample_1thread": f"{n1} pairs"}
+               "sample": f"{ns} seeded pairs of the K=5000 workload (the parity sample), FP64 C oracle (OpenMP)",
+               "seconds": cpu_sec, "value_1thread": n1 / one_sec, "sample_1thread": f"{n1} pairs"}
+
+    dropin = None
+    if not args.no_dropin:
+        dropin = dropin_bench(pf, K, T, ms, step, args)
 
     out = {
         "metric": METRIC,
@@ -241,17 +250,51 @@ def main():
                      "kernel": "pair_kernel (pass 1)", "kernel_ms": k_ms,
                      "flops_per_pair": W_pair, "executed_flops_per_pair": W_pair / 2},
         "cpu_baseline": cpu,
-        "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "ms_pairs": st32["ms_pairs"],
-                     "variant": st32["variant"], "autotune": tune32},
+        "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "seconds": fp32_sec,
+                     "ms_pairs": st32["ms_pairs"], "variant": st32["variant"], "autotune": tune32},
+        "dropin": dropin,
         "autotune": tune,
         "learned": learned,
         "speedup_vs_fp32": value / fp32_value,
+        "speedup_vs_fp32_same_K": Kf == K,
         "clstack_match_pct": 100.0 * float(same.mean()),
         "parity_sample": {"pairs": int(ns), "disagree": int((~same).sum()), "near_tie": int(near),
                           "corr_rel_max": corr_rel},
         "stats": all_stats[-1],
     }
     print(json.dumps(out))
+
+
+def dropin_bench(pf, K, T, ms, step, args):
+    """What a MATLAB caller of the drop-in pays: host pf (complex128,
+    column-major, as the MEX gateway receives it) through clm_clmatrix (H2D,
+    the device job, D2H of the per-pair records) and the host FP64 assembly
+    of clstack, corrstack, shift_equations_map (clm_assemble) and the sparse
+    shift_equations (clm_shift_equations_csc), one timed call after an
+    untimed one."""
+    import torch
+
+    from aspire_amd.clmatrix import assemble, clmatrix_records
+
+    host = pf.permute(2, 1, 0).cpu().numpy().transpose(2, 1, 0)  # n_r x T x K, column-major
+    assert host.flags.f_contiguous
+    rec, stats = clmatrix_records(host, ms, step, precision=args.precision, flags=args.flags)
+    del rec
+    t0 = time.perf_counter()
+    rec, stats = clmatrix_records(host, ms, step, precision=args.precision, flags=args.flags)
+    t1 = time.perf_counter()
+    clstack, corrstack, eqs, emap = assemble(K, T, rec)
+    t2 = time.perf_counter()
+    out = {"seconds": t2 - t0, "pairs_per_s": K * (K - 1) // 2 / (t2 - t0),
+           "clm_clmatrix_s": t1 - t0, "device_job_s": stats["ms_total"] * 1e-3,
+           "transfers_s": (t1 - t0) - stats["ms_total"] * 1e-3, "pf_bytes": int(host.nbytes),
+           "assemble_s": t2 - t1, "shift_equations_nnz": int(eqs.nnz),
+           "note": "host pf -> clm_clmatrix (H2D, device job, D2H of records) -> clm_assemble + "
+                   "clm_shift_equations_csc (FP64 host) -> scipy CSC"}
+    log(f"drop-in: {out}")
+    del clstack, corrstack, eqs, emap, rec, host
+    torch.cuda.empty_cache()
+    return out
 
 
 def configs_bench(args):

@@ -49,3 +49,31 @@ def assert_parity(rep, min_agree=0.999, corr_rel=1e-5):
     assert rep["n_disagree"] <= max(1, int((1 - min_agree) * rep["n"])), rep
     assert rep["corr_rel_max"] <= corr_rel, rep
     assert rep["shift_ok"], rep
+
+
+def oracle_sample(oracle, pf, k1, k2, max_shift, step):
+    """The oracle on the pairs (k1, k2) of ``pf`` (numpy array or column-major
+    device tensor), reading only the images those pairs use.  Returns
+    (P of those images, oracle records, k1 and k2 re-indexed into P)."""
+    imgs = np.unique(np.concatenate([k1, k2]))
+    if type(pf).__module__.startswith("torch"):
+        import torch
+
+        sub = pf[:, :, torch.as_tensor(imgs, device=pf.device)].cpu().numpy()
+    else:
+        sub = np.asarray(pf)[:, :, imgs]
+    sub = sub.astype(np.complex128)
+    kk1 = np.searchsorted(imgs, k1).astype(np.int32)
+    kk2 = np.searchsorted(imgs, k2).astype(np.int32)
+    P, ora = oracle_records(oracle, sub, max_shift, step, kk1, kk2)
+    return P, ora, kk1, kk2
+
+
+def host_subset(rec, slots):
+    """Host copies of per-pair records (numpy arrays or device tensors) at
+    the given equation slots."""
+    out = {}
+    for k, v in rec.items():
+        a = v.cpu().numpy() if hasattr(v, "cpu") else np.asarray(v)
+        out[k] = a[slots]
+    return out
