@@ -6,6 +6,7 @@ if libclm.so is missing or cannot be loaded, every call raises.
 from __future__ import annotations
 
 import ctypes
+import logging
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -31,7 +32,11 @@ EXPORTS = (
     "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
     "clm_pft_device", "clm_pft", "clm_pft_freqs", "clm_estimate_shifts",
     "clm_clmatrix_filtered", "clm_clmatrix_filtered_device", "clm_syncmatrix_vote",
+    "clm_set_log", "clm_clstack_mask",
 )
+
+# void (*)(void *user, int level, const char *msg)
+LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p)
 
 CLM_GROUP_GRAPH = 1
 
@@ -189,9 +194,10 @@ def load():
     L.clm_clmatrix.restype = ctypes.c_int
     L.clm_clmatrix.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
                                ctypes.POINTER(ClmStats)]
-    L.clm_clmatrix_device.restype = ctypes.c_int
-    L.clm_clmatrix_device.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
-                                      ctypes.POINTER(ClmStats)]
+    for name in ("clm_clmatrix_device",):
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp, ctypes.POINTER(ClmStats)]
     for name in ("clm_clmatrix_filtered", "clm_clmatrix_filtered_device"):
         fn = getattr(L, name)
         fn.restype = ctypes.c_int
@@ -233,6 +239,10 @@ def load():
                                       ctypes.c_int32, ctypes.c_double, vp, ctypes.POINTER(ClmLsqrStats)]
     L.clm_syncmatrix_vote.restype = ctypes.c_int
     L.clm_syncmatrix_vote.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ClmVoteStats)]
+    L.clm_set_log.restype = None
+    L.clm_set_log.argtypes = [vp, LOG_FN, vp]
+    L.clm_clstack_mask.restype = ctypes.c_int
+    L.clm_clstack_mask.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ctypes.c_int64)]
     L.clm_pft_freqs.restype = ctypes.c_int
     L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L
@@ -243,8 +253,23 @@ class ClmError(RuntimeError):
     pass
 
 
+def _forward_log(user, level, msg):
+    """clm_set_log callback: the library's verbose reports and warnings go to
+    the ``aspire_amd`` logger."""
+    text = msg.decode(errors="replace") if msg else ""
+    logger = logging.getLogger("aspire_amd")
+    if level:
+        logger.warning(text)
+    else:
+        logger.info(text)
+
+
+_LOG_CB = LOG_FN(_forward_log)  # one C callback object, alive as long as the module
+
+
 class Context:
-    """Owns one clm_ctx (one HIP stream on one device)."""
+    """Owns one clm_ctx (one HIP stream on one device); its verbose reports
+    go to the ``aspire_amd`` logger."""
 
     def __init__(self, device: int = 0):
         L = load()
@@ -253,6 +278,7 @@ class Context:
         if not self._h:
             raise ClmError(f"clm_create(device={device}) failed with code {err.value}")
         self.device = device
+        L.clm_set_log(self._h, _LOG_CB, None)
 
     @property
     def handle(self):

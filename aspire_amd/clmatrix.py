@@ -11,7 +11,9 @@ signature and returns its five outputs in the reference's form
 * ``shift_equations`` scipy.sparse CSC, P x (2K+1), rows in (k1,k2) order,
                   last column = 1-D shift (:722-772);
 * ``shift_equations_map`` K x K float64, equation number of (k1,k2);
-* ``clstack_mask`` K x K zeros (only the reference's debug mode fills it).
+* ``clstack_mask`` K x K: with verbose bit 2 and ``ref_clmatrix`` given, 1
+                  where both lines are within 5 degrees of the reference's
+                  (cryo_clmatrix_cpu.m:583-601,690-700), else zeros.
 
 The pair search runs on the GPU through libclm; there is no CPU path.
 """
@@ -248,10 +250,19 @@ def cryo_clmatrix(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, map_filter
     accepted and ignored (:123), ``max_shift`` defaults to 15, ``shift_step``
     to 1.  ``map_filter_radius`` > 0 (argument 6, :24-29) filters every map
     with cryo_average_clmap before the search (FP64 path; corrstack then
-    holds 1 - the filtered value, as the reference stores it);
-    ``ref_clmatrix``/``ref_shifts_2d`` only feed the reference's debug output
-    and are ignored.  ``precision`` selects the arithmetic of the unfiltered
-    search: "bf16" (default), "fp16", "fp32" or "fp64".
+    holds 1 - the filtered value, as the reference stores it).
+    ``precision`` selects the arithmetic of the unfiltered search: "bf16"
+    (default), "fp16", "fp32" or "fp64".
+
+    ``verbose`` is the reference's bitmask (:155-204).  Bit 1: the library
+    reports the call's shape and stages through the ``aspire_amd`` logger.
+    Bit 2 (detailed debugging): one report per pair (lines, correlation,
+    1-D shift, and the reference line pair and match status when
+    ``ref_clmatrix`` is given, :611-661), and ``clstack_mask`` marks the
+    pairs whose lines are within 5 degrees of ``ref_clmatrix`` (:583-601,
+    :690-700); with ``ref_shifts_2d`` also the true 1-D shift of the pair's
+    estimated lines and the error (:614-636).  Bits 4 and 8 ask the
+    reference for figures; they are not drawn here.
     """
     if hasattr(pf, "shape") and len(pf.shape) >= 2 and pf.shape[1] % 2:
         raise ValueError("n_theta must be even")  # cryo_clmatrix_cpu.m:84-86
@@ -265,7 +276,68 @@ def cryo_clmatrix(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, map_filter
     else:
         shp = np.shape(pf)
         n_theta, K = shp[1], (shp[2] if len(shp) == 3 else 1)
-    if stats.get("small_norm_lines"):
-        log.warning("Ray norm is close to zero for %d lines", stats["small_norm_lines"])
     clstack, corrstack, eqs, emap = assemble(K, n_theta, rec)
-    return clstack, corrstack, eqs, emap, np.zeros((K, K))
+    mask = np.zeros((K, K), order="F")
+    have_ref = ref_clmatrix is not None and np.ndim(ref_clmatrix) == 2
+    if verbose and not have_ref:
+        log.info("Reference clmatrix not found")  # :166-168
+    if verbose & 2 and have_ref:
+        mask, matched = clstack_mask(clstack, ref_clmatrix, n_theta)
+    if verbose & 2:
+        _debug_report(K, n_theta, rec, clstack, corrstack, ref_clmatrix if have_ref else None, mask,
+                      ref_shifts_2d if ref_shifts_2d is not None and np.ndim(ref_shifts_2d) == 2 else None)
+        if have_ref:
+            P = K * (K - 1) // 2
+            log.info("Matched common-lines=%d/%d  (%3.1f%%)", matched, P, 100.0 * matched / max(P, 1))
+    return clstack, corrstack, eqs, emap, mask
+
+
+def clstack_mask(clstack, ref_clmatrix, n_theta):
+    """(mask, matched): libclm's clm_clstack_mask (host FP64) -- 1 at (k1,k2),
+    k1 < k2, where both estimated lines lie within 5 degrees of the
+    reference's, or both within 5 degrees of their antipodes
+    (cryo_clmatrix_cpu.m:583-601)."""
+    cl = np.asfortranarray(clstack, dtype=np.float64)
+    ref = np.asfortranarray(ref_clmatrix, dtype=np.float64)
+    if cl.shape != ref.shape or cl.ndim != 2 or cl.shape[0] != cl.shape[1]:
+        raise ValueError("clstack and ref_clmatrix must both be K x K")
+    K = cl.shape[0]
+    mask = np.zeros((K, K), order="F")
+    matched = ctypes.c_int64(0)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = _lib.load().clm_clstack_mask(int(K), int(n_theta), vp(cl), vp(ref), vp(mask), ctypes.byref(matched))
+    if rc:
+        raise _lib.ClmError(f"clm_clstack_mask failed ({rc})")
+    return mask, int(matched.value)
+
+
+def _debug_report(K, n_theta, rec, clstack, corrstack, ref, mask, ref_shifts):
+    """Per-pair reports of verbose bit 2 (cryo_clmatrix_cpu.m:611-661)."""
+    L = n_theta // 2
+    dtheta = np.pi / L
+    e = 0
+    for k1 in range(K):
+        for k2 in range(k1 + 1, K):
+            log.info("Finding common-line between projections [k1 k2]=[ %d %d ]", k1 + 1, k2 + 1)
+            log.info("\t Common lines:")
+            log.info("\t \t clstack= [ %3d %3d ]", clstack[k1, k2], clstack[k2, k1])
+            if ref is not None:
+                log.info("\t \t ref    = [ %3d %3d ]", ref[k1, k2], ref[k2, k1])
+                log.info("\t \t status = %s (%s than 5.00 degrees)",
+                         "MATCHED" if mask[k1, k2] else "NOT MATCHED", "less" if mask[k1, k2] else "more")
+            log.info("\t Correlation:")
+            log.info("\t \t est    =  %9.6f", corrstack[k1, k2])
+            log.info("\t Shifts:")
+            log.info("\t \t est  shift= %6.4f", rec["shift"][e])
+            if ref_shifts is not None:
+                a, b = (clstack[k1, k2] - 1) * dtheta, (clstack[k2, k1] - 1) * dtheta
+                dx1, dy1 = ref_shifts[k1, 0], ref_shifts[k1, 1]
+                dx2, dy2 = ref_shifts[k2, 0], ref_shifts[k2, 1]
+                if b < np.pi - 1.0e-13:
+                    true1d = np.sin(a) * dx1 + np.cos(a) * dy1 - np.sin(b) * dx2 - np.cos(b) * dy2
+                else:
+                    b -= np.pi
+                    true1d = -np.sin(a) * dx1 - np.cos(a) * dy1 - np.sin(b) * dx2 - np.cos(b) * dy2
+                log.info("\t \t True shift= %6.4f", true1d)
+                log.info("\t \t shift err = %3.2e", rec["shift"][e] - true1d)
+            e += 1

@@ -256,6 +256,17 @@ int plan_stack(clm_ctx* c, const clm_params* p, StackJob& j) {
     if (prec < CLM_FP32 || prec > CLM_FP64_REF) return fail(c, CLM_E_ARG, "unknown precision %d", prec);
     if (p->pf_dtype != CLM_C128 && p->pf_dtype != CLM_C64) return fail(c, CLM_E_ARG, "unknown pf dtype %d", p->pf_dtype);
     if (g.Rp < 0) return fail(c, CLM_E_UNSUP, "n_r=%d exceeds the supported 129", g.n_r);
+    {
+        // The margin sets both the pass-1 keep window (tie_margin/2 beyond the
+        // certified bound) and the FP64 re-score window below the FP32 best.
+        // Below a few times the FP32 re-score's own error (rays are unit norm,
+        // so (2Rp+16)·2^-24 in correlation units) the FP64 stage could miss
+        // the true maximum; NaN or negative would re-score nothing.
+        const double floor_tm = 4.0 * (2.0 * g.Rp + 16.0) * 0x1p-24;
+        if (!(p->tie_margin >= floor_tm) || !std::isfinite(p->tie_margin))
+            return fail(c, CLM_E_ARG, "tie_margin %g is below the FP32 re-score error floor %.3g (or not finite)",
+                        p->tie_margin, floor_tm);
+    }
     j.opk = prec == CLM_BF16_RESCORE ? OPK_BF16 : prec == CLM_FP16_RESCORE ? OPK_F16 : OPK_F32;
     j.exact = prec == CLM_FP64_REF;
     j.use64 = prec != CLM_FP32;
@@ -915,8 +926,8 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
 
 // Device job of n stacks (results in device memory).  With `graph`, the job
 // is captured once and the graph replayed while the signature repeats.
-int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st,
-            const StackJob* extra = nullptr) {
+int run_job_core(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st,
+                 const StackJob* extra) {
     Plan pl;
     int rc = prepare(c, pl, n, ps, ios, true, extra);
     if (rc) return rc;
@@ -953,6 +964,56 @@ int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bo
     HIPCHK(c, hipStreamSynchronize(c->stream));
     rc = add_stats(c, pl, st);
     if (st) st->ms_total = event_ms(c->ev[0], c->ev[3]);
+    return rc;
+}
+
+void log_line(clm_ctx* c, int level, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->log_fn(c->log_user, level, buf);
+}
+
+// What verbose bit 1 reports (the reference prints per-pair progress,
+// cryo_clmatrix_cpu.m:744-749): the call's shape and, once the device job is
+// done, its stages and counts.  The job is one device launch sequence with no
+// host wait, so the report follows its completion.  Rays of near-zero norm
+// are reported whatever the verbose bits, as cryo_raynormalize.m:20-22 warns.
+void log_job(clm_ctx* c, int n, const clm_params* ps, const clm_stats* st) {
+    if (!c->log_fn || !st) return;
+    bool progress = false;
+    for (int i = 0; i < n; i++) progress = progress || (ps[i].verbose & 1);
+    static const char* pname[] = {"FP32", "BF16 + re-score", "FP16 + re-score", "FP64"};
+    if (progress) {
+        for (int i = 0; i < n; i++) {
+            const clm_params& p = ps[i];
+            const int S = (int)std::ceil(2.0 * p.max_shift / p.shift_step + 1.0);
+            log_line(c, 0, "%s%sK=%d n_r=%d n_theta=%d max_shift=%g shift_step=%g (%d shifts), %s",
+                     n > 1 ? "stack " : "cryo_clmatrix: ", n > 1 ? (std::to_string(i) + ": ").c_str() : "", p.K,
+                     p.n_r, p.n_theta, p.max_shift, p.shift_step, S,
+                     p.precision >= 0 && p.precision <= 3 ? pname[p.precision] : "?");
+        }
+        log_line(c, 0, "%lld pairs in %.2f ms (%.4g pairs/s): preprocessing %.2f ms, contraction %.2f ms, "
+                       "re-score %.2f ms; kernel variant %d, %d evaluated shifts",
+                 (long long)st->pairs, st->ms_total, st->ms_total > 0 ? st->pairs / (st->ms_total * 1e-3) : 0.0,
+                 st->ms_prep, st->ms_pairs, st->ms_rescore, st->variant, st->n_eval_shifts);
+        log_line(c, 0, "%lld cells re-scored in FP32, %lld candidates in FP64; %lld uncertain pairs (%lld refined "
+                       "with FP32 operands), %lld finished by the FP64 exhaustive search",
+                 (long long)st->cells_rescored, (long long)st->fp64_candidates, (long long)st->uncertain_pairs,
+                 (long long)st->refined_pairs, (long long)st->fallback_pairs);
+    }
+    if (st->small_norm_lines)
+        log_line(c, 1, "Ray norm is close to zero for %lld lines (norm < 1e-13)", (long long)st->small_norm_lines);
+}
+
+int run_job(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st,
+            const StackJob* extra = nullptr) {
+    clm_stats local;
+    if (!st && c->log_fn) st = &local;
+    const int rc = run_job_core(c, n, ps, ios, graph, st, extra);
+    if (rc == CLM_OK) log_job(c, n, ps, st);
     return rc;
 }
 
@@ -1344,6 +1405,44 @@ int clm_clmatrix_filtered_device(clm_ctx* c, const clm_params* p, int32_t map_fi
     ex.filt = map_filter_radius;
     const clm_stack_io io{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
     return run_job(c, 1, p, &io, false, st, &ex);
+}
+
+void clm_set_log(clm_ctx* c, clm_log_fn fn, void* user) {
+    if (!c) return;
+    c->log_fn = fn;
+    c->log_user = user;
+}
+
+int clm_clstack_mask(int32_t K, int32_t n_theta, const double* clstack, const double* ref_clmatrix, double* mask,
+                     int64_t* matched) {
+    if (K < 0 || n_theta <= 0 || n_theta % 2 || (K > 0 && (!clstack || !ref_clmatrix || !mask))) return CLM_E_ARG;
+    // cryo_clmatrix_cpu.m:583-601 with n_theta := n_theta/2 (:119): lines are
+    // matched when both lie within 5 degrees of the reference's (chord
+    // distance on the unit circle), or both within 5 degrees of its antipodes
+    const int L = n_theta / 2;
+    const double max_angle = 5.0 / 180.0 * M_PI;
+    const double tol = 2.0 * std::sin(max_angle / 2.0) + 1.0e-10;
+    const double dth = 2.0 * M_PI / (2.0 * L);
+    auto chord = [&](double a, double b, bool flip) {
+        const double ta = dth * (a - 1.0) + (flip ? M_PI : 0.0), tb = dth * (b - 1.0);
+        return std::hypot(std::cos(ta) - std::cos(tb), std::sin(ta) - std::sin(tb));
+    };
+    int64_t n = 0;
+    const size_t KK = (size_t)K * K;
+    for (size_t i = 0; i < KK; i++) mask[i] = 0.0;
+    for (int k1 = 0; k1 < K; k1++)
+        for (int k2 = k1 + 1; k2 < K; k2++) {
+            const size_t e12 = (size_t)k2 * K + k1, e21 = (size_t)k1 * K + k2;  // (k1,k2), (k2,k1) column-major
+            const double l1 = clstack[e12], l2 = clstack[e21], t1 = ref_clmatrix[e12], t2 = ref_clmatrix[e21];
+            const bool same = chord(l1, t1, false) <= tol && chord(l2, t2, false) <= tol;
+            const bool flip = chord(l1, t1, true) <= tol && chord(l2, t2, true) <= tol;
+            if (same || flip) {
+                mask[e12] = 1.0;
+                n++;
+            }
+        }
+    if (matched) *matched = n;
+    return CLM_OK;
 }
 
 int clm_assemble(int32_t K, const int32_t* cl1, const int32_t* cl2, const double* corr, double* clstack,

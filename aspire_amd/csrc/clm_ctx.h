@@ -66,6 +66,9 @@ struct clm_ctx {
     std::vector<unsigned long long> gsig;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // verbose reports and warnings (clm_set_log)
+    clm_log_fn log_fn = nullptr;
+    void* log_user = nullptr;
 };
 
 namespace clmh {

@@ -111,6 +111,20 @@ void clm_destroy(clm_ctx *ctx);
 const char *clm_last_error(const clm_ctx *ctx);
 
 /*
+ * Reports of the verbose argument (cryo_clmatrix_cpu.m:155-204).  With
+ * verbose bit 1 set in clm_params, each finished call sends the call's shape
+ * and then its stages (preprocessing, contraction, re-score: times, pair and
+ * candidate counts) to fn with level 0; the reference prints per-pair
+ * progress instead (:744-749), which a single device job has no host point
+ * for.  Lines of near-zero norm (cryo_raynormalize.m:20-22) are sent with
+ * level 1 whatever the verbose bits.  fn runs on the calling thread before
+ * the entry point returns (the MEX gateway passes mexPrintf through it).
+ * fn = NULL turns reports off (the default).
+ */
+typedef void (*clm_log_fn)(void *user, int level, const char *msg);
+void clm_set_log(clm_ctx *ctx, clm_log_fn fn, void *user);
+
+/*
  * The hot path: all K(K-1)/2 pairs of cryo_clmatrix_cpu.m:296-751
  * (preprocessing :96,:277-293 included).
  *
@@ -271,6 +285,19 @@ int clm_shiftnet_adam(clm_ctx *ctx, float *d_master, uint16_t *d_w,
                       float *d_m, float *d_v, const float *d_grad,
                       int32_t step, float lr, float beta1, float beta2,
                       float eps);
+
+/*
+ * clstack_mask, the fifth output, as the reference fills it under verbose
+ * bit 2 when ref_clmatrix is given (cryo_clmatrix_cpu.m:583-601,690-700):
+ * mask(k1,k2) = 1 for k1 < k2 when both estimated lines lie within 5 degrees
+ * of the reference's lines (or both within 5 degrees of their antipodes),
+ * else 0.  clstack, ref_clmatrix and mask are K x K column-major doubles
+ * (1-based line indices in [1, n_theta]); matched (optional) receives the
+ * number of matched pairs.  Host FP64.
+ */
+int clm_clstack_mask(int32_t K, int32_t n_theta, const double *clstack,
+                     const double *ref_clmatrix, double *mask,
+                     int64_t *matched);
 
 /*
  * Host-side FP64 assembly of the reference outputs from per-pair records

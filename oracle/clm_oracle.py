@@ -202,6 +202,37 @@ def cryo_clmatrix_cpu(pf, NK=None, verbose=0, max_shift=15, shift_step=1.0, nthr
     return clstack, corrstack, eqs, emap, np.zeros((K, K))
 
 
+def clstack_mask(clstack, ref_clmatrix, n_theta):
+    """clstack_mask of the reference's detailed-debugging mode
+    (``cryo_clmatrix_cpu.m:577-601,690-700``), in its own arithmetic: with
+    n_theta := n_theta/2 (:119), alpha = 2*pi*i/(2 n_theta), a pair is
+    matched when |e^{alpha(l-1)} - e^{alpha(t-1)}| <= 2 sin(max_angle/2) +
+    1e-10 (max_angle = 5 degrees) for both lines, or the same with e^{i pi}
+    applied to both estimated lines.  Returns (mask, matched)."""
+    cl = np.asarray(clstack, dtype=np.float64)
+    ref = np.asarray(ref_clmatrix, dtype=np.float64)
+    K = cl.shape[0]
+    L = n_theta // 2
+    max_angle = 5 / 180 * np.pi
+    angle_tol = 2 * np.sin(max_angle / 2) + 1.0e-10
+    alpha = 2 * np.pi * 1j / (2 * L)
+    PI = 4 * np.arctan(1.0)
+    mask = np.zeros((K, K))
+    matched = 0
+    for k1 in range(K):
+        for k2 in range(k1 + 1, K):
+            tcl1, tcl2 = ref[k1, k2], ref[k2, k1]
+            l1, l2 = cl[k1, k2], cl[k2, k1]
+            d1s = abs(np.exp(alpha * (l1 - 1)) - np.exp(alpha * (tcl1 - 1)))
+            d2s = abs(np.exp(alpha * (l2 - 1)) - np.exp(alpha * (tcl2 - 1)))
+            d1f = abs(np.exp(alpha * (l1 - 1) + 1j * PI) - np.exp(alpha * (tcl1 - 1)))
+            d2f = abs(np.exp(alpha * (l2 - 1) + 1j * PI) - np.exp(alpha * (tcl2 - 1)))
+            if (d1s <= angle_tol and d2s <= angle_tol) or (d1f <= angle_tol and d2f <= angle_tol):
+                mask[k1, k2] = 1
+                matched += 1
+    return mask, matched
+
+
 # ------------------------------------------------------ polar Fourier transform
 def pft_freqs(n_r: int, n_theta: int) -> np.ndarray:
     """freqs of cryo_pft.m:52-58: (n_r*n_theta/2) x 2."""

@@ -194,3 +194,17 @@ def test_gpu_clean_data_refinement(oracle, flags):
     rep = compare(oracle, P, ora, sub, 4, 0.5, k1[sel], k2[sel])
     say(rep["agree"], rep["n_disagree"], rep["corr_rel_max"])
     assert_parity(rep)
+
+
+@pytest.mark.parametrize("tm", [0.0, -1e-3, float("nan"), float("inf"), 1e-7])
+def test_gpu_tie_margin_rejected(stack_c1, tm):
+    """A tie margin below the FP32 re-score's error floor, negative, NaN or
+    infinite would let the FP64 stage skip the true maximum (or re-score
+    nothing): the library refuses it with an argument error."""
+    from aspire_amd import _lib
+
+    say("case tie-margin", tm)
+    with pytest.raises(_lib.ClmError, match="tie_margin"):
+        clmatrix_records(stack_c1.pf[:, :, :6], 2, 1, precision="bf16", tie_margin=tm)
+    rec, _ = clmatrix_records(stack_c1.pf[:, :, :6], 2, 1, precision="bf16", tie_margin=1e-4)
+    assert rec["cl1"].size == 15
