@@ -33,6 +33,7 @@ EXPORTS = (
     "clm_pft_device", "clm_pft", "clm_pft_freqs", "clm_estimate_shifts",
     "clm_clmatrix_filtered", "clm_clmatrix_filtered_device", "clm_syncmatrix_vote",
     "clm_set_log", "clm_clstack_mask",
+    "clm_sync3n_triangles_scores", "clm_sync3n_pairs_probabilities", "clm_sync3n_signs_times_v",
 )
 
 # void (*)(void *user, int level, const char *msg)
@@ -164,6 +165,17 @@ class ClmVoteStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ClmSync3nStats(ctypes.Structure):
+    _fields_ = [
+        ("triangles", ctypes.c_int64),
+        ("ms_layout", ctypes.c_float),
+        ("ms_kernel", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -239,6 +251,13 @@ def load():
                                       ctypes.c_int32, ctypes.c_double, vp, ctypes.POINTER(ClmLsqrStats)]
     L.clm_syncmatrix_vote.restype = ctypes.c_int
     L.clm_syncmatrix_vote.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ClmVoteStats)]
+    S3 = ctypes.POINTER(ClmSync3nStats)
+    L.clm_sync3n_triangles_scores.restype = ctypes.c_int
+    L.clm_sync3n_triangles_scores.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, S3]
+    L.clm_sync3n_pairs_probabilities.restype = ctypes.c_int
+    L.clm_sync3n_pairs_probabilities.argtypes = [vp, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_double), vp, vp, S3]
+    L.clm_sync3n_signs_times_v.restype = ctypes.c_int
+    L.clm_sync3n_signs_times_v.argtypes = [vp, ctypes.c_int32, vp, vp, ctypes.c_int32, ctypes.c_int32, vp, vp, S3]
     L.clm_set_log.restype = None
     L.clm_set_log.argtypes = [vp, LOG_FN, vp]
     L.clm_clstack_mask.restype = ctypes.c_int

@@ -4,7 +4,7 @@
 clm_api.hip`` (the engine: context, orchestration and its kernels),
 ``shiftnet.hip`` (the shift network), ``pft.hip`` (the polar Fourier
 transform), ``shifts.hip`` (the 2-D shift least squares), ``vote.hip`` (the
-voting synchronization matrix) and ``pair_variants.hip`` once per (operand
+voting synchronization matrix), ``sync3n.hip`` (the sync3n triangle kernels) and ``pair_variants.hip`` once per (operand
 type, shift-grid symmetry), and links them into ``aspire_amd/libclm.so``.
 """
 from __future__ import annotations
@@ -22,7 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 # instantiations, one object per (operand type, shift-grid symmetry)
 UNITS = [(os.path.join(CSRC, "clm_api.hip"), []), (os.path.join(CSRC, "shiftnet.hip"), []),
          (os.path.join(CSRC, "pft.hip"), []), (os.path.join(CSRC, "shifts.hip"), []),
-         (os.path.join(CSRC, "vote.hip"), [])] + [
+         (os.path.join(CSRC, "vote.hip"), []), (os.path.join(CSRC, "sync3n.hip"), [])] + [
     (os.path.join(CSRC, "pair_variants.hip"), [f"-DCLM_VARIANT_OP={op}", f"-DCLM_VARIANT_SYM={sym}"])
     for op in ("bf16", "f16", "f32") for sym in (1, 0)]
 ARCH = "gfx950"

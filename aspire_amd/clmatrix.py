@@ -13,7 +13,7 @@ signature and returns its five outputs in the reference's form
 * ``shift_equations_map`` K x K float64, equation number of (k1,k2);
 * ``clstack_mask`` K x K: with verbose bit 2 and ``ref_clmatrix`` given, 1
                   where both lines are within 5 degrees of the reference's
-                  (cryo_clmatrix_cpu.m:583-601,690-700), else zeros.
+                  (cryo_clmatrix_cpu.m:567-617,697-698), else zeros.
 
 The pair search runs on the GPU through libclm; there is no CPU path.
 """
@@ -258,10 +258,10 @@ def cryo_clmatrix(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, map_filter
     reports the call's shape and stages through the ``aspire_amd`` logger.
     Bit 2 (detailed debugging): one report per pair (lines, correlation,
     1-D shift, and the reference line pair and match status when
-    ``ref_clmatrix`` is given, :611-661), and ``clstack_mask`` marks the
-    pairs whose lines are within 5 degrees of ``ref_clmatrix`` (:583-601,
-    :690-700); with ``ref_shifts_2d`` also the true 1-D shift of the pair's
-    estimated lines and the error (:614-636).  Bits 4 and 8 ask the
+    ``ref_clmatrix`` is given, :689-720), and ``clstack_mask`` marks the
+    pairs whose lines are within 5 degrees of ``ref_clmatrix`` (:567-617,
+    :697-698); with ``ref_shifts_2d`` also the true 1-D shift of the pair's
+    estimated lines and the error (:665-685).  Bits 4 and 8 ask the
     reference for figures; they are not drawn here.
     """
     if hasattr(pf, "shape") and len(pf.shape) >= 2 and pf.shape[1] % 2:
@@ -296,7 +296,7 @@ def clstack_mask(clstack, ref_clmatrix, n_theta):
     """(mask, matched): libclm's clm_clstack_mask (host FP64) -- 1 at (k1,k2),
     k1 < k2, where both estimated lines lie within 5 degrees of the
     reference's, or both within 5 degrees of their antipodes
-    (cryo_clmatrix_cpu.m:583-601)."""
+    (cryo_clmatrix_cpu.m:567-617)."""
     cl = np.asfortranarray(clstack, dtype=np.float64)
     ref = np.asfortranarray(ref_clmatrix, dtype=np.float64)
     if cl.shape != ref.shape or cl.ndim != 2 or cl.shape[0] != cl.shape[1]:
@@ -312,7 +312,7 @@ def clstack_mask(clstack, ref_clmatrix, n_theta):
 
 
 def _debug_report(K, n_theta, rec, clstack, corrstack, ref, mask, ref_shifts):
-    """Per-pair reports of verbose bit 2 (cryo_clmatrix_cpu.m:611-661)."""
+    """Per-pair reports of verbose bit 2 (cryo_clmatrix_cpu.m:689-720)."""
     L = n_theta // 2
     dtheta = np.pi / L
     e = 0

@@ -174,6 +174,34 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 // and call; uncertain pairs beyond it go to the FP64 exhaustive kernel.
 constexpr int UCAP = 32768;
 
+// 64-bit counters per group after the 16 globals (see enqueue_job)
+constexpr int GSTRIDE = 8;
+// pass-1 work queues, one per XCD (workgroup b runs on XCD b % 8)
+constexpr int NQ = 8;
+
+// Split k1-major pass-1 tiles into NQ contiguous runs of about equal pair
+// counts and append the NQ+1 run bounds (ints, padded to whole int4s) after
+// the tiles: the pair kernel reads them at tiles + ntiles.
+void append_queue_bounds(std::vector<int4>& h) {
+    const int ntiles = (int)h.size();
+    long long tot = 0;
+    for (int i = 0; i < ntiles; i++) tot += h[i].z & 15;
+    int b[NQ + 1];
+    b[0] = 0;
+    long long acc = 0;
+    int q = 1;
+    for (int i = 0; i < ntiles && q < NQ; i++) {
+        acc += h[i].z & 15;
+        while (q < NQ && acc * NQ >= tot * q) b[q++] = i + 1;
+    }
+    while (q < NQ) b[q++] = ntiles;
+    b[NQ] = ntiles;
+    static_assert(NQ == 8, "bounds packing");
+    h.push_back(make_int4(b[0], b[1], b[2], b[3]));
+    h.push_back(make_int4(b[4], b[5], b[6], b[7]));
+    h.push_back(make_int4(b[8], 0, 0, 0));
+}
+
 // FP64 exhaustive search of items (pairs): the first EX_CAP items split their
 // shift grid over up to EX_SJ workgroups each (few items still fill the
 // GPU), the rest take one workgroup per item.
@@ -475,8 +503,8 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         }
     }
     if ((rc = upload(c, c->table, pl.table))) return rc;
-    // shared buffers: counters (16 globals + 4 per group), lists, pass-2 tiles
-    if ((rc = ensure(c, c->ctr, (16 + 4 * pl.groups.size()) * sizeof(unsigned long long)))) return rc;
+    // shared buffers: counters (16 globals + GSTRIDE per group), lists, pass-2 tiles
+    if ((rc = ensure(c, c->ctr, (16 + GSTRIDE * pl.groups.size()) * sizeof(unsigned long long)))) return rc;
     if ((rc = ensure(c, c->ulist, (size_t)std::max<long long>(ubase, 1) * sizeof(SlotRef)))) return rc;
     if ((rc = ensure(c, c->fblist, (size_t)std::max<long long>(pl.P, 1) * sizeof(SlotRef)))) return rc;
     long long gmax = 0;
@@ -528,6 +556,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
             TileCache& tc = c->tcache.front();
             tc.key = key;
             tc.ntiles = (int)h.size();
+            append_queue_bounds(h);
             if ((rc = upload(c, tc.tiles, h))) return rc;
             HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released on return
         }
@@ -638,8 +667,9 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
     if (phases & PHASE_A) {
     // ctr (u64): 0 small-norm lines, 1 FP64 candidates, 4 fallback count (int),
     // 6 cells re-scored, 7 second-pass candidates; per group at 16+4g:
-    // 0 pass-1 tile counter, 1 uncertain count, 2 pass-2 tile counter, 3 pass-2 tiles
-    if ((rc = enqueue_fill(c, ctr, 2 * (16 + 4 * (long long)pl.groups.size()), 0))) return rc;
+    // 0 pass-1 tile counter (one queue), 1 uncertain count, 2 pass-2 tile counter,
+    // 3 pass-2 tiles, 4-7 pass-1 tile counters of the NQ XCD queues (int each)
+    if ((rc = enqueue_fill(c, ctr, 2 * (16 + GSTRIDE * (long long)pl.groups.size()), 0))) return rc;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = enqueue_prep(c, pl))) return rc;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -671,10 +701,11 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
             if ((rc = enqueue_exact(c, ea, G.P, S))) return rc;
             continue;
         }
-        unsigned long long* cg = ctr + 16 + 4 * gi;
+        unsigned long long* cg = ctr + 16 + GSTRIDE * gi;
         PairArgs pa{};
         pa.stacks = dtab;
         pa.counter = (int*)(cg + 0);
+        pa.nq = 1;
         pa.erecs = nullptr;
         pa.CPP = G.CPP;
         pa.lag = G.lag;
@@ -695,6 +726,11 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
             max_tiles = G.tc->ntiles;
         }
         const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, max_tiles);
+        if (!G.sel && grid >= (unsigned)NQ) {
+            pa.nq = NQ;
+            pa.qbeg = (const int*)(pa.tiles + pa.ntiles);
+            pa.counter = (int*)(cg + 4);
+        }
         hipLaunchKernelGGL(G.fn, dim3(grid), dim3(64 * G.W), G.lds, c->stream, pa);
         HIPCHK(c, hipGetLastError());
     }
@@ -703,7 +739,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
     for (size_t gi = 0; gi < pl.groups.size(); gi++) {
         const Group& G = pl.groups[gi];
         if (G.exact || G.twin_of >= 0) continue;
-        unsigned long long* cg = ctr + 16 + 4 * gi;
+        unsigned long long* cg = ctr + 16 + GSTRIDE * gi;
         SlotRef* ul = (SlotRef*)c->ulist.p + G.ubase;
         RescoreArgs ra{};
         ra.stacks = dtab;
@@ -740,8 +776,8 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
             // then take the FP32 second pass below
             const Group& G0 = pl.groups[gi0];
             const Group& T = pl.groups[G0.twin];
-            unsigned long long* c0 = ctr + 16 + 4 * gi0;
-            unsigned long long* cT = ctr + 16 + 4 * (size_t)G0.twin;
+            unsigned long long* c0 = ctr + 16 + GSTRIDE * gi0;
+            unsigned long long* cT = ctr + 16 + GSTRIDE * (size_t)G0.twin;
             const SlotRef* ul0 = (const SlotRef*)c->ulist.p + G0.ubase;
             const int shift = T.first - G0.first;
             hipLaunchKernelGGL(build_pass2_kernel, dim3((unsigned)std::min<long long>((G0.P + 255) / 256, 1024)),
@@ -782,7 +818,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
             gi = (size_t)G0.twin;
         }
         const Group& G = pl.groups[gi];
-        unsigned long long* cg = ctr + 16 + 4 * gi;
+        unsigned long long* cg = ctr + 16 + GSTRIDE * gi;
         SlotRef* ul = (SlotRef*)c->ulist.p + G.ubase;
         RescoreArgs ra{};
         ra.stacks = dtab;
@@ -840,7 +876,7 @@ int enqueue_job(clm_ctx* c, const Plan& pl, bool timed, int phases = PHASE_ALL) 
 // Add a finished plan's counters to st (which the caller cleared).
 int add_stats(clm_ctx* c, const Plan& pl, clm_stats* st) {
     if (!st || pl.P == 0) return CLM_OK;
-    std::vector<unsigned long long> h(16 + 4 * pl.groups.size());
+    std::vector<unsigned long long> h(16 + GSTRIDE * pl.groups.size());
     HIPCHK(c, hipMemcpy(h.data(), c->ctr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     auto as_int = [](unsigned long long x) { return (int64_t)(int)(unsigned)(x & 0xffffffffu); };
     st->small_norm_lines += (int64_t)h[0];
@@ -852,8 +888,8 @@ int add_stats(clm_ctx* c, const Plan& pl, clm_stats* st) {
         const Group& G = pl.groups[gi];
         if (G.exact && !G.sel) st->fallback_pairs += G.P;
         else if (!G.exact && G.twin_of < 0) {
-            st->uncertain_pairs += as_int(h[16 + 4 * gi + 1]);
-            if (G.twin >= 0) st->refined_pairs += as_int(h[16 + 4 * gi + 1]);
+            st->uncertain_pairs += as_int(h[16 + GSTRIDE * gi + 1]);
+            if (G.twin >= 0) st->refined_pairs += as_int(h[16 + GSTRIDE * gi + 1]);
         }
     }
     return CLM_OK;
@@ -1174,22 +1210,30 @@ int autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int sample_pairs
                     h.push_back(make_int4((int)k1, (int)k2, cnt, (int)pair_slot(k1, k2, K)));
                     taken += cnt;
                 }
+            const int ntiles = (int)h.size();
+            append_queue_bounds(h);
             if ((rc = upload(c, c->tune_tiles, h))) return rc;
             HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released on return
-            if ((rc = enqueue_fill(c, c->ctr.p, 2 * 20, 0))) return rc;
+            if ((rc = enqueue_fill(c, c->ctr.p, 2 * (16 + GSTRIDE), 0))) return rc;
             if ((rc = enqueue_prep(c, pl))) return rc;
             PairArgs pa{};
             pa.stacks = (const StackDev*)c->table.p;
             pa.tiles = (const int4*)c->tune_tiles.p;
-            pa.ntiles = (int)h.size();
+            pa.ntiles = ntiles;
             pa.counter = (int*)((unsigned long long*)c->ctr.p + 16);
+            pa.nq = 1;
             pa.CPP = G.CPP;
             pa.lag = G.lag;
             pa.prio = (p->reserved & 4) ? 0 : 1;
-            const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, (long long)h.size());
+            const unsigned grid = (unsigned)std::min<long long>((long long)c->n_cu * G.occ, (long long)ntiles);
+            if (grid >= (unsigned)NQ) {  // the same XCD queues as the measured job
+                pa.nq = NQ;
+                pa.qbeg = (const int*)(pa.tiles + ntiles);
+                pa.counter = (int*)((unsigned long long*)c->ctr.p + 16 + 4);
+            }
             float ms = 0.f;
             for (int rep = 0; rep < 2; rep++) {  // first launch warms caches and code
-                if ((rc = enqueue_fill(c, pa.counter, 1, 0))) return rc;
+                if ((rc = enqueue_fill(c, pa.counter, pa.nq, 0))) return rc;
                 HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
                 hipLaunchKernelGGL(G.fn, dim3(grid), dim3(64 * G.W), G.lds, c->stream, pa);
                 HIPCHK(c, hipGetLastError());
@@ -1285,8 +1329,9 @@ void clm_destroy(clm_ctx* c) {
                     &c->w_list, &c->w_plist, &c->w_tiles, &c->w_counts, &c->w_shiftv};
     for (DevBuf* b : wb) release(*b);
     DevBuf* pb[] = {&c->pft_ax, &c->pft_ay, &c->pft_xt, &c->pft_img, &c->pft_out,
-                    &c->ls_pack, &c->ls_u, &c->ls_part, &c->ls_vec, &c->ls_tab, &c->ls_state,
-                    &c->vt_m, &c->vt_mt, &c->vt_pk, &c->vt_cs};
+                    &c->ls_pack, &c->ls_u, &c->ls_part, &c->ls_vec, &c->ls_tab, &c->ls_state, &c->ls_tiles,
+                    &c->vt_m, &c->vt_mt, &c->vt_pk, &c->vt_cs,
+                    &c->s3_rs, &c->s3_vs, &c->s3_ss, &c->s3_tiles, &c->s3_thr, &c->s3_hist};
     for (DevBuf* b : pb) release(*b);
     DevBuf* bufs[] = {&c->table, &c->ctr, &c->ulist, &c->erecs, &c->plist, &c->tiles2, &c->fblist, &c->pf,
                       &c->out_cl1, &c->out_cl2, &c->out_j, &c->out_corr, &c->out_shift};
@@ -1416,7 +1461,7 @@ void clm_set_log(clm_ctx* c, clm_log_fn fn, void* user) {
 int clm_clstack_mask(int32_t K, int32_t n_theta, const double* clstack, const double* ref_clmatrix, double* mask,
                      int64_t* matched) {
     if (K < 0 || n_theta <= 0 || n_theta % 2 || (K > 0 && (!clstack || !ref_clmatrix || !mask))) return CLM_E_ARG;
-    // cryo_clmatrix_cpu.m:583-601 with n_theta := n_theta/2 (:119): lines are
+    // cryo_clmatrix_cpu.m:567-617 with n_theta := n_theta/2 (:119): lines are
     // matched when both lie within 5 degrees of the reference's (chord
     // distance on the unit circle), or both within 5 degrees of its antipodes
     const int L = n_theta / 2;

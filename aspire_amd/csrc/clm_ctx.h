@@ -54,11 +54,17 @@ struct clm_ctx {
     clmh::DevBuf sn_h1, sn_h2, sn_z, sn_y, sn_dy, sn_dz, sn_dz2, sn_dz1;
     // polar Fourier transform: per-axis tables, staged images, host-form buffers
     clmh::DevBuf pft_ax, pft_ay, pft_xt, pft_img, pft_out;
-    // shift least squares: packed lines, u, row partial sums, 2K vectors, tables, scalar state
-    clmh::DevBuf ls_pack, ls_u, ls_part, ls_vec, ls_tab, ls_state;
+    // shift least squares: packed lines, u, per-tile sums, 2K vectors and the tiles' partial
+    // A'u sums, tables, scalar state, pair tiles
+    clmh::DevBuf ls_pack, ls_u, ls_part, ls_vec, ls_tab, ls_state, ls_tiles;
+    int ls_tiles_K = 0, ls_ntiles = 0;  // the pair tiles held in ls_tiles are for this K
     // voting: clstack as 16-bit tables, row-major and transposed; per-pair histogram
     // peaks; cos and sin of the line-difference angles
     clmh::DevBuf vt_m, vt_mt, vt_pk, vt_cs;
+    // sync3n: symmetric copies of R (9 planes), of v and of the pair scores;
+    // pair tiles (for s3_nb blocks per side), histogram thresholds and counts
+    clmh::DevBuf s3_rs, s3_vs, s3_ss, s3_tiles, s3_thr, s3_hist;
+    int s3_nb = 0;
     // autotuned variants: shape key -> (ct, W, lag)
     std::vector<std::pair<unsigned long long, int4>> tuned;
     clmh::DevBuf tune_tiles;

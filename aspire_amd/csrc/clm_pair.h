@@ -100,7 +100,9 @@ struct PairArgs {
     const int4* tiles;     // {k1, k2_first | list offset, count | stack << 4, slot | -1}
     int ntiles;            // tile count, or
     const int* ntiles_dev; // (when set) tile count in device memory
-    int* counter;          // next tile (persistent work queue)
+    int* counter;          // next tile (persistent work queue); per queue when nq > 1
+    const int* qbeg;       // nq > 1: queue q serves tiles [qbeg[q], qbeg[q+1])
+    int nq;                // work queues: 1, or one per XCD (workgroup b starts on queue b % nq)
     const int4* plist;     // explicit pairs {k2, slot, u, 0}
     EmitRec* erecs;        // indexed by u (second pass)
     int CPP;               // waves per pair (column chunks); same for all stacks
@@ -179,9 +181,31 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
     }
     const bool lag = a.lag && lag_wave;
     const int ntiles = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    // XCD-aware queues: dispatch places workgroup b on XCD b % 8, and queue
+    // q holds a contiguous run of k1-major tiles, so the tiles of one image
+    // k1 (which all stream its A-stack) meet in one XCD's L2.  A workgroup
+    // drains its own XCD's queue, then helps the next ones.
+    int cq = a.nq > 1 ? (int)(blockIdx.x % (unsigned)a.nq) : 0;
+    int left = a.nq > 1 ? a.nq : 1;
 
     for (;;) {
-        if (tid == 0) s_i[0] = atomicAdd(a.counter, 1);
+        if (tid == 0) {
+            int t = ntiles;
+            if (a.nq > 1) {
+                while (left > 0) {
+                    const int u = a.qbeg[cq] + atomicAdd(a.counter + cq, 1);
+                    if (u < a.qbeg[cq + 1]) {
+                        t = u;
+                        break;
+                    }
+                    cq = cq + 1 == a.nq ? 0 : cq + 1;
+                    left--;
+                }
+            } else {
+                t = atomicAdd(a.counter, 1);
+            }
+            s_i[0] = t;
+        }
         if (tid >= 1 && tid < 32) s_i[tid] = 0;
         if (tid < 32) { s_unc[tid] = 0; s_pm[tid] = -INFINITY; }
         __syncthreads();

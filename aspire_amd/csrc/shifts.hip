@@ -16,13 +16,18 @@
 // solve is a fixed sequence of launches with no host round trip (after the
 // tests pass, the remaining launches return at once).
 //
-// Memory per iteration (K = 5000, 12.5M pairs): the packed lines (4 B/pair)
-// are read three times and u (8 B/pair) read three times and written once:
-// about 0.5 GB, HBM-bound work with every reduction in a fixed order
-// (results do not depend on scheduling).
+// One pass over the pairs per iteration: u' = A v - (alpha/beta) u is formed
+// and, in the same pass, A'u' (A' is linear, so the normalization by the new
+// beta = |u'| is applied afterwards to the 2K-vector).  Memory per iteration
+// (K = 5000, 12.5M pairs): the packed lines (4 B/pair) and u (8 B/pair) are
+// read once and u written once, 20 B/pair, about 0.25 GB, plus the 2K-long
+// partial sums of the 64 x 256-pair tiles; HBM-bound work with every
+// reduction in a fixed order (results do not depend on scheduling).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "clm_ctx.h"
 
@@ -38,7 +43,6 @@ struct State {
 
 __device__ inline long long first(long long k1, long long K) { return k1 * K - k1 * (k1 + 1) / 2; }
 
-constexpr int NS = 16;  // k1 residue classes of the column part of A'u (partial sums)
 
 // ta[c] = (sin a, cos a), a = (c-1)*pi/L; tb[c] = (-sin b', -cos b', sign):
 // b = (c-1)*pi/L, b' = b or b - pi, sign = +1 (b < pi) or -1 (b >= pi), for
@@ -83,59 +87,109 @@ __device__ inline double block_sum(double v, double* sh) {
     return r;
 }
 
-// pack[e] = cl1 | cl2 << 16 (both 2L+1 when either is outside [0, 2L]);
-// u[e] = b[e];  part[k1] = sum over row k1 of b^2.
-__global__ __launch_bounds__(256) void init_kernel(int K, int L, const int* cl1, const int* cl2, const double* b,
-                                                   unsigned* pack, double* u, double* part) {
-    __shared__ double sh[256];
-    const int k1 = blockIdx.x;
-    const long long e0 = first(k1, K);
-    const int n = K - 1 - k1;
-    double s = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) {
-        const long long e = e0 + j;
-        const int a = cl1[e], c2 = cl2[e];
-        const bool ok = a >= 0 && a <= 2 * L && c2 >= 0 && c2 <= 2 * L;
-        const unsigned z = (unsigned)(2 * L + 1);
-        pack[e] = ok ? ((unsigned)a | ((unsigned)c2 << 16)) : (z | (z << 16));
-        const double be = b[e];
-        u[e] = be;
-        s += be * be;
-    }
-    s = block_sum<256>(s, sh);
-    if (threadIdx.x == 0) part[k1] = s;
+// Pair tiles: rows k1 in [TR*kb, TR*kb + TR), columns k2 in [TC*cb, TC*cb + TC);
+// a tile exists when it holds a pair k1 < k2 < K.
+constexpr int TR = 64, TC = 256;
+
+__host__ __device__ inline bool tile_exists(int K, int kb, int cb) {
+    const int kmax = min(TC * cb + TC, K) - 1;  // largest k2 of the column block
+    return TR * kb < kmax;
 }
 
-// u <- A v - alpha * u / beta_prev (u holds the previous unnormalized vector,
-// whose norm is beta_prev); part[k1] = sum over row k1 of the new u^2.
-__global__ __launch_bounds__(256) void av_kernel(int K, const unsigned* pack, const double2* ta, const double* tb,
-                                                 const double* v, const State* S, double* u, double* part) {
-    __shared__ double sh[256];
-    if (S->done) return;
-    const int k1 = blockIdx.x;
-    const long long e0 = first(k1, K);
-    const int n = K - 1 - k1;
-    const double ca = S->alpha / S->beta;
-    const double v0 = v[2 * k1], v1 = v[2 * k1 + 1];
-    double s = 0.0;
-    for (int j = threadIdx.x; j < n; j += 256) {
-        const long long e = e0 + j;
-        const int k2 = k1 + 1 + j;
-        const unsigned pk = pack[e];
-        const double2 A = ta[pk & 0xffff];
-        const double* B = tb + 3 * (pk >> 16);
-        const double sg = B[2];
-        const double av = sg * A.x * v0 + sg * A.y * v1 + B[0] * v[2 * k2] + B[1] * v[2 * k2 + 1];
-        const double un = av - ca * u[e];
-        u[e] = un;
-        s += un * un;
-    }
-    s = block_sum<256>(s, sh);
-    if (threadIdx.x == 0) part[k1] = s;
+// Fixed-order sum over the 64 lanes (xor butterfly); every lane gets it.
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
 }
 
-// beta = sqrt(sum of part); first call (init): bnorm = beta; later: the
-// running Frobenius estimate of A.
+// One pass over tile t's pairs (row-major, pair e = first(k1) + k2 - k1 - 1):
+//   INIT: pack[e] = cl1 | cl2 << 16 (both 2L+1 when either is outside
+//         [0, 2L]) and u[e] = b[e];
+//   else: u[e] <- (A v)[e] - (alpha/beta) u[e]  (u holds the previous
+//         unnormalized vector, whose norm is beta);
+// and, of the new u: normp[t] = sum of squares; rowp[cb][2k1 + c] = the
+// tile's row part of A'u (the k1 coefficients), colp[kb][2k2 + c] = its
+// column part (the k2 coefficients).  A thread owns one column and walks the
+// tile's rows in order; row sums go through fixed wave butterflies and then
+// the four waves in order.
+template <bool INIT>
+__global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __restrict__ tiles,
+                                                   const int* __restrict__ cl1, const int* __restrict__ cl2,
+                                                   const double* __restrict__ b, unsigned* __restrict__ pack,
+                                                   const double2* __restrict__ ta, const double* __restrict__ tb,
+                                                   const double* __restrict__ v, const State* S, double* u,
+                                                   double* normp, double* rowp, double* colp) {
+    __shared__ double s_row[TR][4][2];
+    __shared__ double s_sq[4];
+    if (!INIT && S->done) return;
+    const int2 tl = tiles[blockIdx.x];
+    const int kb = tl.x, cb = tl.y;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int k2 = TC * cb + t;
+    const double ca = INIT ? 0.0 : S->alpha / S->beta;
+    double va0 = 0.0, va1 = 0.0;
+    if (!INIT && k2 < K) {
+        va0 = v[2 * k2];
+        va1 = v[2 * k2 + 1];
+    }
+    double c0 = 0.0, c1 = 0.0, sq = 0.0;
+    const unsigned z = (unsigned)(2 * L + 1);
+#pragma unroll 8
+    for (int r = 0; r < TR; r++) {
+        const int k1 = TR * kb + r;
+        double r0 = 0.0, r1 = 0.0;
+        if (k1 < K && k2 > k1 && k2 < K) {
+            const long long e = first(k1, K) + (k2 - k1 - 1);
+            unsigned pk;
+            double un;
+            if (INIT) {
+                const int a = cl1[e], c2 = cl2[e];
+                const bool ok = a >= 0 && a <= 2 * L && c2 >= 0 && c2 <= 2 * L;
+                pk = ok ? ((unsigned)a | ((unsigned)c2 << 16)) : (z | (z << 16));
+                pack[e] = pk;
+                un = b[e];
+            } else {
+                pk = pack[e];
+            }
+            const double2 A = ta[pk & 0xffff];
+            const double* B = tb + 3 * (pk >> 16);
+            const double sg = B[2];
+            if (!INIT) {
+                const double av = sg * A.x * v[2 * k1] + sg * A.y * v[2 * k1 + 1] + B[0] * va0 + B[1] * va1;
+                un = av - ca * u[e];
+            }
+            u[e] = un;
+            sq += un * un;
+            r0 = sg * A.x * un;
+            r1 = sg * A.y * un;
+            c0 += B[0] * un;
+            c1 += B[1] * un;
+        }
+        r0 = wave_sum(r0);
+        r1 = wave_sum(r1);
+        if (lane == 0) {
+            s_row[r][wave][0] = r0;
+            s_row[r][wave][1] = r1;
+        }
+    }
+    sq = wave_sum(sq);
+    if (lane == 0) s_sq[wave] = sq;
+    if (k2 < K) {
+        colp[(size_t)kb * 2 * K + 2 * k2] = c0;
+        colp[(size_t)kb * 2 * K + 2 * k2 + 1] = c1;
+    }
+    __syncthreads();
+    if (t < 2 * TR) {
+        const int r = t >> 1, c = t & 1, k1 = TR * kb + r;
+        if (k1 < K)
+            rowp[(size_t)cb * 2 * K + 2 * k1 + c] = ((s_row[r][0][c] + s_row[r][1][c]) + s_row[r][2][c]) + s_row[r][3][c];
+    }
+    if (t == 0) normp[blockIdx.x] = ((s_sq[0] + s_sq[1]) + s_sq[2]) + s_sq[3];
+}
+
+// beta = sqrt(sum of the tiles' squares, in tile order); first call (init):
+// bnorm = beta; later: the running Frobenius estimate of A.
 __global__ __launch_bounds__(1024) void beta_kernel(int n, const double* part, State* S, int init) {
     __shared__ double sh[1024];
     if (S->done) return;
@@ -157,76 +211,25 @@ __global__ __launch_bounds__(1024) void beta_kernel(int n, const double* part, S
     }
 }
 
-// A'u in one launch, in fixed-order partial sums:
-//  blocks [0, K): vt_row[2k..2k+1] = sum over pairs (k, k2 > k) of the k1
-//    coefficients * u (one image, its pairs contiguous);
-//  blocks [K, K + ceil(K/16)*NS): vt_col[s][2k..2k+1] = sum over pairs
-//    (k1 < k, k), k1 = s*16 + q (mod 16*NS), of the k2 coefficients * u: 16
-//    consecutive images per block, so for one k1 the 16 pairs are
-//    consecutive; lane q of image i walks its residue class.
-__global__ __launch_bounds__(256) void atu_kernel(int K, const unsigned* pack, const double2* ta, const double* tb,
-                                                  const double* u, const State* S, double* vt_row, double* vt_col) {
-    __shared__ double sh0[256], sh1[256];
-    if (S->done) return;
-    double s0 = 0.0, s1 = 0.0;
-    if ((int)blockIdx.x < K) {
-        const int k = blockIdx.x;
-        const long long e0 = first(k, K);
-        const int n = K - 1 - k;
-        for (int j = threadIdx.x; j < n; j += 256) {
-            const unsigned pk = pack[e0 + j];
-            const double2 A = ta[pk & 0xffff];
-            const double sg = tb[3 * (pk >> 16) + 2];
-            const double ue = u[e0 + j];
-            s0 += sg * A.x * ue;
-            s1 += sg * A.y * ue;
-        }
-        s0 = block_sum<256>(s0, sh0);
-        s1 = block_sum<256>(s1, sh0);
-        if (threadIdx.x == 0) {
-            vt_row[2 * k] = s0;
-            vt_row[2 * k + 1] = s1;
-        }
-        return;
-    }
-    const int cb = blockIdx.x - K;
-    const int sp = cb % NS, kb = cb / NS;
-    const int i = threadIdx.x & 15, q = threadIdx.x >> 4;
-    const int k = kb * 16 + i;
-    if (k < K) {
-        for (int k1 = sp * 16 + q; k1 < k; k1 += 16 * NS) {
-            const long long e = first(k1, K) + (k - k1 - 1);
-            const unsigned pk = pack[e];
-            const double* B = tb + 3 * (pk >> 16);
-            const double ue = u[e];
-            s0 += B[0] * ue;
-            s1 += B[1] * ue;
-        }
-    }
-    sh0[threadIdx.x] = s0;
-    sh1[threadIdx.x] = s1;
-    __syncthreads();
-    if (q == 0 && k < K) {
-        double a0 = 0.0, a1 = 0.0;
-        for (int r = 0; r < 16; r++) {
-            a0 += sh0[r * 16 + i];
-            a1 += sh1[r * 16 + i];
-        }
-        vt_col[(size_t)sp * 2 * K + 2 * k] = a0;
-        vt_col[(size_t)sp * 2 * K + 2 * k + 1] = a1;
-    }
-}
-
-__device__ inline double atu_value(int n2, const double* vt_row, const double* vt_col, int i) {
-    double t = vt_row[i];
-#pragma unroll
-    for (int sp = 0; sp < NS; sp++) t += vt_col[(size_t)sp * n2 + i];
+// (A'u)[i] for entry i of the 2K-vector: the row parts of the tiles in
+// image i/2's row block, then the column parts of the tiles in its column
+// block, each in block order.
+__device__ inline double atu_value(int K, const double* rowp, const double* colp, int i) {
+    const int k = i >> 1, n2 = 2 * K;
+    const int kb = k / TR, cbk = k / TC;
+    const int ncb = (K + TC - 1) / TC, nkb = (K + TR - 1) / TR;
+    double t = 0.0;
+    for (int cb = 0; cb < ncb; cb++)
+        if (tile_exists(K, kb, cb)) t += rowp[(size_t)cb * n2 + i];
+    for (int b = 0; b < nkb; b++)
+        if (tile_exists(K, b, cbk)) t += colp[(size_t)b * n2 + i];
     return t;
 }
 
 // First direction: v = A'u / |A'u| with u = b/beta; w = v; x = 0.
-__global__ __launch_bounds__(1024) void vinit_kernel(int n2, const double* vt_row, const double* vt_col, State* S,
+__global__ __launch_bounds__(1024) void vinit_kernel(int K, const double* rowp, const double* colp, State* S,
                                                      double* v, double* w, double* x) {
+    const int n2 = 2 * K;
     __shared__ double sh[1024];
     if (S->done) {
         for (int i = threadIdx.x; i < n2; i += 1024) x[i] = 0.0;
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(1024) void vinit_kernel(int n2, const double* vt_ro
     const double ib = 1.0 / S->beta;
     double s = 0.0;
     for (int i = threadIdx.x; i < n2; i += 1024) {
-        const double t = atu_value(n2, vt_row, vt_col, i) * ib;
+        const double t = atu_value(K, rowp, colp, i) * ib;
         v[i] = t;
         s += t * t;
     }
@@ -273,15 +276,16 @@ __global__ __launch_bounds__(1024) void vinit_kernel(int n2, const double* vt_ro
 // beta = 0 (b in the range of A): v and alpha stay, the rotation below makes
 // the last update of x and phibar becomes 0.  stepc applies the update of
 // the iteration whose tests pass, as the last one.
-__global__ __launch_bounds__(256) void stepa_kernel(int n2, const double* vt_row, const double* vt_col,
-                                                    const State* S, double* v, double* part) {
+__global__ __launch_bounds__(256) void stepa_kernel(int K, const double* rowp, const double* colp, const State* S,
+                                                    double* v, double* part) {
+    const int n2 = 2 * K;
     __shared__ double sh[256];
     if (S->done || !(S->beta > 0.0)) return;
     const double beta = S->beta, ib = 1.0 / beta;
     const int i = blockIdx.x * 256 + threadIdx.x;
     double t = 0.0;
     if (i < n2) {
-        t = atu_value(n2, vt_row, vt_col, i) * ib - beta * v[i];
+        t = atu_value(K, rowp, colp, i) * ib - beta * v[i];
         v[i] = t;
     }
     const double sq = block_sum<256>(t * t, sh);
@@ -365,19 +369,36 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     HIPCHK(c, hipSetDevice(c->device));
     const long long P = (long long)K * (K - 1) / 2;
     const int L = n_theta / 2, n2 = 2 * K;
+    const int ncb = (K + TC - 1) / TC, nkb = (K + TR - 1) / TR;
     int rc;
+    if (c->ls_tiles_K != K) {
+        // the pair tiles, row blocks in order (a row block's tiles share its v entries)
+        std::vector<int2> h;
+        for (int kb = 0; kb < nkb; kb++)
+            for (int cb = 0; cb < ncb; cb++)
+                if (tile_exists(K, kb, cb)) h.push_back(make_int2(kb, cb));
+        if ((rc = ensure(c, c->ls_tiles, h.size() * sizeof(int2)))) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->ls_tiles.p, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released on return
+        c->ls_tiles_K = K;
+        c->ls_ntiles = (int)h.size();
+    }
+    const int ntiles = c->ls_ntiles;
+    const int nstep = (n2 + 255) / 256;
     if ((rc = ensure(c, c->ls_pack, (size_t)P * 4)) || (rc = ensure(c, c->ls_u, (size_t)P * 8)) ||
-        (rc = ensure(c, c->ls_part, (size_t)K * 8)) || (rc = ensure(c, c->ls_vec, (size_t)n2 * (3 + NS) * 8)) ||
+        (rc = ensure(c, c->ls_part, (size_t)std::max(ntiles, nstep) * 8)) ||
+        (rc = ensure(c, c->ls_vec, (size_t)n2 * (2 + ncb + nkb) * 8)) ||
         (rc = ensure(c, c->ls_tab, (size_t)(2 * L + 2) * 5 * 8)) || (rc = ensure(c, c->ls_state, sizeof(State))))
         return rc;
     double* vec = (double*)c->ls_vec.p;
-    double *v = vec, *w = vec + n2, *vt_row = vec + 2 * n2, *vt_col = vec + 3 * n2;  // vt_col: NS x 2K
+    double *v = vec, *w = vec + n2, *rowp = vec + 2 * n2, *colp = rowp + (size_t)ncb * n2;  // [ncb][2K], [nkb][2K]
     double2* ta = (double2*)c->ls_tab.p;
     double* tb = (double*)c->ls_tab.p + 2 * (2 * L + 2);
     State* S = (State*)c->ls_state.p;
     unsigned* pack = (unsigned*)c->ls_pack.p;
     double* u = (double*)c->ls_u.p;
     double* part = (double*)c->ls_part.p;
+    const int2* tiles = (const int2*)c->ls_tiles.p;
     State h{};
     h.damp = damp;
     h.tol = tol;
@@ -387,20 +408,18 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, hipMemcpyAsync(S, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     tables_kernel<<<(2 * L + 2 + 255) / 256, 256, 0, c->stream>>>(L, ta, tb);
-    init_kernel<<<K, 256, 0, c->stream>>>(K, L, d_cl1, d_cl2, d_shift1d, pack, u, part);
-    beta_kernel<<<1, 1024, 0, c->stream>>>(K, part, S, 1);
-    const int natu = K + (K + 15) / 16 * NS;
-    const int nstep = (n2 + 255) / 256;  // <= K: its partial sums fit in part
-    atu_kernel<<<natu, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row, vt_col);
-    vinit_kernel<<<1, 1024, 0, c->stream>>>(n2, vt_row, vt_col, S, v, w, d_x);
+    pass_kernel<true><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, d_cl1, d_cl2, d_shift1d, pack, ta, tb, v, S, u, part,
+                                                     rowp, colp);
+    beta_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, S, 1);
+    vinit_kernel<<<1, 1024, 0, c->stream>>>(K, rowp, colp, S, v, w, d_x);
     HIPCHK(c, hipGetLastError());
     // launches after the stopping tests pass return at once; the done flag is
     // read back every 8 iterations so a converged solve stops issuing them
     for (int it = 0; it < maxit; it++) {
-        av_kernel<<<K, 256, 0, c->stream>>>(K, pack, ta, tb, v, S, u, part);
-        beta_kernel<<<1, 1024, 0, c->stream>>>(K, part, S, 0);
-        atu_kernel<<<natu, 256, 0, c->stream>>>(K, pack, ta, tb, u, S, vt_row, vt_col);
-        stepa_kernel<<<nstep, 256, 0, c->stream>>>(n2, vt_row, vt_col, S, v, part);
+        pass_kernel<false><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, nullptr, nullptr, nullptr, pack, ta, tb, v, S, u,
+                                                          part, rowp, colp);
+        beta_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, S, 0);
+        stepa_kernel<<<nstep, 256, 0, c->stream>>>(K, rowp, colp, S, v, part);
         stepb_kernel<<<1, 1024, 0, c->stream>>>(nstep, part, S);
         stepc_kernel<<<nstep, 256, 0, c->stream>>>(n2, S, v, w, d_x);
         if (it % 8 == 7 && it + 1 < maxit) {

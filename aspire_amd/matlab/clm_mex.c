@@ -11,7 +11,7 @@
  * GPU, then clm_assemble and clm_shift_equations_csc on the host in FP64.
  * precision: 0 FP32, 1 BF16 + re-score (default), 2 FP16 + re-score, 3 FP64.
  * clstack_mask is filled by clm_clstack_mask when verbose has bit 2 and a
- * K x K ref_clmatrix is given (cryo_clmatrix_cpu.m:583-601,690-700), else 0.
+ * K x K ref_clmatrix is given (cryo_clmatrix_cpu.m:567-617,697-698), else 0.
  * verbose bit 1 prints libclm's reports with mexPrintf; lines of near-zero
  * norm raise the warning clm:raynormalize (cryo_raynormalize.m:20-22).
  *

@@ -8,7 +8,7 @@ function [clstack,corrstack,shift_equations,shift_equations_map,clstack_mask]=..
 %   NK                is accepted and ignored, as cryo_clmatrix_cpu.m:123 does.
 %   verbose           bit 1: libclm's stage reports; bit 2 with ref_clmatrix:
 %                     clstack_mask marks the pairs whose lines are within 5
-%                     degrees of ref_clmatrix (cryo_clmatrix_cpu.m:583-601).
+%                     degrees of ref_clmatrix (cryo_clmatrix_cpu.m:567-617).
 %   map_filter_radius > 0 filters every correlation map with
 %                     cryo_average_clmap before the search (FP64).
 %   ref_shifts_2d     only feeds the reference's per-pair debug printing; it is

@@ -441,6 +441,29 @@ def _cmd_train(args):
     print(json.dumps(info))
 
 
+def _cmd_calibrate(net: "ShiftNet", args) -> dict:
+    """The gate: on a held-out stack (seed outside the training seeds, images
+    of the learned workload's model: 129 px, shifts U[-12,12], 49-shift full
+    grid), the smallest window peak such that among the pairs the window
+    settles at least ``target`` agree with the full-grid search
+    (:func:`calibrate_gate`).  Stored in the weights' meta as ``gate``."""
+    from . import sim
+    from .clmatrix import clmatrix_records
+
+    st = sim.make_stack(K=args.K, n=129, n_r=65, n_theta=T, snr=args.snr, max_shift_2d=12.0, shift_kind="uniform",
+                        seed=args.seed, device="cuda")
+    full, _ = clmatrix_records(st.pf, 12.0, 0.5, precision="bf16")
+    full = {k: v.cpu().numpy() for k, v in full.items()}
+    prop = net.propose(st.pf)
+    gate, table = calibrate_gate(st.pf, prop, full, 12.0, 0.5, 1.0, target=args.target)
+    _, ws = clmatrix_windowed(st.pf, prop, 12.0, 0.5, 1.0, gate)
+    info = dict(gate=gate, K=args.K, snr=args.snr, seed=args.seed, target=args.target, table=table,
+                window_stats=ws)
+    net.meta["gate"] = float(gate)
+    net.meta["gate_calibration"] = {k: v for k, v in info.items() if k not in ("table", "window_stats")}
+    return info
+
+
 def main(argv=None):
     import argparse
 
@@ -456,6 +479,12 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--out", default=WEIGHTS)
+    c = sub.add_parser("calibrate", help="pick the window gate on a held-out stack and store it in the weights")
+    c.add_argument("--weights", default=WEIGHTS)
+    c.add_argument("--K", type=int, default=1000)
+    c.add_argument("--snr", type=float, default=1 / 4)
+    c.add_argument("--seed", type=int, default=5101)
+    c.add_argument("--target", type=float, default=0.99)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--K", type=int, default=5000)
@@ -465,6 +494,12 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "train":
         _cmd_train(args)
+    elif args.cmd == "calibrate":
+        import json
+
+        net = ShiftNet.load(args.weights)
+        print(json.dumps(_cmd_calibrate(net, args)))
+        net.save(args.weights)
     elif args.cmd == "evaluate":
         import json
 

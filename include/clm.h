@@ -288,7 +288,7 @@ int clm_shiftnet_adam(clm_ctx *ctx, float *d_master, uint16_t *d_w,
 
 /*
  * clstack_mask, the fifth output, as the reference fills it under verbose
- * bit 2 when ref_clmatrix is given (cryo_clmatrix_cpu.m:583-601,690-700):
+ * bit 2 when ref_clmatrix is given (cryo_clmatrix_cpu.m:567-617,697-698):
  * mask(k1,k2) = 1 for k1 < k2 when both estimated lines lie within 5 degrees
  * of the reference's lines (or both within 5 degrees of their antipodes),
  * else 0.  clstack, ref_clmatrix and mask are K x K column-major doubles
@@ -418,6 +418,52 @@ typedef struct clm_vote_stats {
 int clm_syncmatrix_vote(clm_ctx *ctx, int32_t K, int32_t n_theta,
                         const int32_t *d_cl1, const int32_t *d_cl2,
                         double *d_S, clm_vote_stats *stats);
+
+/*
+ * sync3n triangle kernels (the other consumer of the common lines,
+ * abinitio/C1/cryo_abinitio_C1_worker.m:116-147), on relative rotations
+ * d_R: 3 x 3 x N-choose-2 doubles in MATLAB layout (pair (i,j), i<j, 0-based,
+ * at 9*e with e = (2N-i-1)i/2 + j-i-1, each matrix column-major).  Every
+ * triangle i<j<k is matched in its four J-configurations and scored as in
+ * the reference; per-pair sums add a pair's triangles in ascending order of
+ * the third image (the reference's single-thread order).  Device pointers;
+ * N in [3, 46340].  The kernels use a symmetric copy of R (9 N^2 doubles).
+ *
+ * clm_sync3n_triangles_scores replaces
+ *   [cum_scores, scores_hist] = triangles_scores_mex(N, Rij, n_intervals, ~, 1, ~)
+ * (abinitio/C1/sync3n/triangles_scores_mex.c:1-346, called from
+ * triangles_scores.m:62): d_cum (N-choose-2, or NULL) gets each pair's sum
+ * of its triangles' scores, d_hist (n_intervals) the histogram of all
+ * 3 N-choose-3 scores, bins as the reference forms them.
+ *
+ * clm_sync3n_pairs_probabilities replaces
+ *   [ln_f_ind, ln_f_arb] = pairs_probabilities_mex(N, Rij, P2, A, a, B, b, x0, ~, ~)
+ * (pairs_probabilities_mex.c:1-339); prm (host) = {P2, A, a, B, b, x0}.
+ *
+ * clm_sync3n_signs_times_v replaces
+ *   v2 = signs_times_v_mex(N, Rij, v, n_eigs, scores_as_entries, pairs_scores, ~)
+ * (signs_times_v_mex.c:1-329), the product of the power method
+ * (cryo_sync3n_Jsync_power_method.m:62): d_v and d_v2 are N-choose-2 x n_eigs
+ * column-major, n_eigs in [1, 8]; scores_as_entries 0 (signs), 1 (triangle
+ * scores) or 2 (products of d_pair_scores, N-choose-2).
+ */
+typedef struct clm_sync3n_stats {
+    int64_t triangles; /* N-choose-3                                 */
+    float ms_layout;   /* symmetric copies of R (and v), tables      */
+    float ms_kernel;   /* the triangle kernel                        */
+} clm_sync3n_stats;
+
+int clm_sync3n_triangles_scores(clm_ctx *ctx, int32_t N, const double *d_R,
+                                int32_t n_intervals, double *d_cum,
+                                double *d_hist, clm_sync3n_stats *stats);
+int clm_sync3n_pairs_probabilities(clm_ctx *ctx, int32_t N, const double *d_R,
+                                   const double *prm, double *d_ln_f_ind,
+                                   double *d_ln_f_arb, clm_sync3n_stats *stats);
+int clm_sync3n_signs_times_v(clm_ctx *ctx, int32_t N, const double *d_R,
+                             const double *d_v, int32_t n_eigs,
+                             int32_t scores_as_entries,
+                             const double *d_pair_scores, double *d_v2,
+                             clm_sync3n_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,15 @@ def lib():
         L.clmo_syncmatrix_vote.restype = None
         L.clmo_syncmatrix_vote.argtypes = [_f64p, ctypes.c_int, ctypes.c_double, _f64p, ctypes.c_int]
         L.clmo_max_threads.restype = ctypes.c_int
+        L.s3o_triangles_scores.restype = ctypes.c_int
+        L.s3o_triangles_scores.argtypes = [ctypes.c_int64, _f64p, ctypes.c_int, ctypes.c_void_p, _f64p, ctypes.c_int]
+        L.s3o_pairs_probabilities.restype = ctypes.c_int
+        L.s3o_pairs_probabilities.argtypes = [ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, ctypes.c_int]
+        L.s3o_signs_times_v.restype = ctypes.c_int
+        L.s3o_signs_times_v.argtypes = [ctypes.c_int64, _f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        _f64p, ctypes.c_int]
+        L.s3o_triangle.restype = ctypes.c_int
+        L.s3o_triangle.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p]
         L.clmo_pft_freqs.restype = None
         L.clmo_pft_freqs.argtypes = [ctypes.c_int, ctypes.c_int, _f64p]
         L.clmo_pft.restype = None
@@ -204,7 +213,7 @@ def cryo_clmatrix_cpu(pf, NK=None, verbose=0, max_shift=15, shift_step=1.0, nthr
 
 def clstack_mask(clstack, ref_clmatrix, n_theta):
     """clstack_mask of the reference's detailed-debugging mode
-    (``cryo_clmatrix_cpu.m:577-601,690-700``), in its own arithmetic: with
+    (``cryo_clmatrix_cpu.m:567-617,697-698``), in its own arithmetic: with
     n_theta := n_theta/2 (:119), alpha = 2*pi*i/(2 n_theta), a pair is
     matched when |e^{alpha(l-1)} - e^{alpha(t-1)}| <= 2 sin(max_angle/2) +
     1e-10 (max_angle = 5 degrees) for both lines, or the same with e^{i pi}
@@ -280,3 +289,60 @@ def vote_pair(clstack, L, k1, k2, _flat=None) -> np.ndarray:
     r = np.zeros(4)
     lib().clmo_vote_pair(_cl_colmajor(clstack) if _flat is None else _flat, int(K), float(L), int(k1), int(k2), r)
     return r.reshape(2, 2)
+
+
+# ------------------------------------------------------ sync3n triangle kernels
+def _rpairs(R):
+    """R_pairs (3 x 3 x N-choose-2, MATLAB layout) as a flat column-major array."""
+    R = np.asarray(R, dtype=np.float64)
+    return np.ascontiguousarray(R.transpose(2, 1, 0)).ravel()
+
+
+def _n_of(R):
+    P = np.shape(R)[2]
+    N = int(round((1 + np.sqrt(1 + 8 * P)) / 2))
+    assert N * (N - 1) // 2 == P
+    return N
+
+
+def triangles_scores(R, n_intervals=100, nthreads=1, cum=True):
+    """[cumulated_scores, scores_histogram] of triangles_scores_mex (sync3n_oracle.c)."""
+    N = _n_of(R)
+    P = N * (N - 1) // 2
+    c = np.zeros(P) if cum else None
+    h = np.zeros(n_intervals)
+    lib().s3o_triangles_scores(N, _rpairs(R), int(n_intervals),
+                               c.ctypes.data_as(ctypes.c_void_p) if cum else None, h, int(nthreads))
+    return c, h
+
+
+def pairs_probabilities(R, P2, A, a, B, b, x0, nthreads=1):
+    """[ln_f_ind, ln_f_arb] of pairs_probabilities_mex (sync3n_oracle.c)."""
+    N = _n_of(R)
+    P = N * (N - 1) // 2
+    ind, arb = np.zeros(P), np.zeros(P)
+    prm = np.array([P2, A, a, B, b, x0], dtype=np.float64)
+    lib().s3o_pairs_probabilities(N, _rpairs(R), prm, ind, arb, int(nthreads))
+    return ind, arb
+
+
+def signs_times_v(R, v, scores_as_entries=0, pair_scores=None, nthreads=1):
+    """signs_matrix * v of signs_times_v_mex (sync3n_oracle.c); v is P x n_eigs."""
+    N = _n_of(R)
+    P = N * (N - 1) // 2
+    v = np.asarray(v, dtype=np.float64).reshape(P, -1)
+    ne = v.shape[1]
+    vf = np.ascontiguousarray(v.T).ravel()
+    out = np.zeros(P * ne)
+    ps = None if pair_scores is None else np.ascontiguousarray(pair_scores, dtype=np.float64)
+    lib().s3o_signs_times_v(N, _rpairs(R), vf, ne, int(scores_as_entries),
+                            ps.ctypes.data_as(ctypes.c_void_p) if ps is not None else None, out, int(nthreads))
+    return out.reshape(ne, P).T
+
+
+def triangle(Rij, Rjk, Rik):
+    """(best configuration, c[4], s[3]) of one triangle."""
+    c, s = np.zeros(4), np.zeros(3)
+    f = lambda M: np.ascontiguousarray(np.asarray(M, dtype=np.float64).T).ravel()
+    b = lib().s3o_triangle(f(Rij), f(Rjk), f(Rik), c, s)
+    return int(b), c, s

@@ -1,6 +1,6 @@
 """clstack_mask (the fifth output under verbose bit 2 with ref_clmatrix):
 libclm's host FP64 clm_clstack_mask against the oracle's restatement of
-cryo_clmatrix_cpu.m:577-601, and against what the geometry fixes: lines
+cryo_clmatrix_cpu.m:567-617, and against what the geometry fixes: lines
 equal to the reference match; every line 4 degrees off still matches and 6
 degrees off does not (max_angle is 5 degrees); both lines on their
 antipodes match, one alone does not; an unset line (0) matches only a
