@@ -300,10 +300,20 @@ def clmatrix_windowed(pf, prop, max_shift=12.0, shift_step=0.5, window=1.0, gate
 
 
 def calibrate_gate(pf, prop, full_rec, max_shift=12.0, shift_step=0.5, window=1.0, target=0.999,
-                   precision="bf16"):
-    """Smallest gate on the window peak such that, among the pairs the
-    window settles, at least ``target`` agree with the full-grid result
-    ``full_rec`` (host arrays).  Returns (gate, table of candidates)."""
+                   precision="bf16", rots=None, max_detect_loss=0.005):
+    """The window gate from a stack whose full-grid result ``full_rec``
+    (host arrays) is known.  Returns (gate, table of candidates).
+
+    With the true rotations ``rots`` (a simulated stack): the smallest gate
+    such that the combined result (window where its peak is at or above the
+    gate, full grid elsewhere) finds at least (1 - max_detect_loss) as many
+    true common lines (within 5 degrees, comparecl's test) as the full grid
+    alone.  Without them: the smallest gate such that at least ``target`` of
+    the pairs the window settles agree with the full grid.  Table rows:
+    (gate, fraction of on-grid pairs settled, agreement with the full grid
+    among them, detection of the combined result / of the full grid)."""
+    from . import sim
+
     rec, _ = clmatrix_windowed(pf, prop, max_shift, shift_step, window, gate=-1e30, precision=precision)
     w = {k: v.cpu().numpy() for k, v in rec.items()}
     same = (w["cl1"] == full_rec["cl1"]) & (w["cl2"] == full_rec["cl2"]) & (w["j"] == full_rec["j"])
@@ -311,16 +321,30 @@ def calibrate_gate(pf, prop, full_rec, max_shift=12.0, shift_step=0.5, window=1.
     # grid; at gate g it settles those of them with window corr >= g, and
     # the rest are searched exhaustively (and then agree by construction)
     keep = w["exhaustive"] == 0
-    same, corr = same[keep], w["corr"][keep]
+    corr = w["corr"][keep]
     order = np.argsort(-corr)
-    cum_bad = np.cumsum(~same[order])
     n_acc = np.arange(1, corr.size + 1)
-    ok = cum_bad <= (1 - target) * n_acc
+    cum_bad = np.cumsum(~same[keep][order])
+    if rots is not None:
+        K = rots.shape[2]
+        ref = sim.clmatrix_cheat(rots, T)
+        k1, k2 = np.triu_indices(K, 1)
+        t1, t2 = ref[k1, k2], ref[k2, k1]
+        ok_full = sim.matched_lines(full_rec["cl1"], full_rec["cl2"], t1, t2, T, 5)
+        ok_win = sim.matched_lines(w["cl1"], w["cl2"], t1, t2, T, 5)
+        base = float(ok_full.sum())
+        gain = np.cumsum(ok_win[keep][order].astype(np.int64) - ok_full[keep][order].astype(np.int64))
+        detect = (base + gain) / max(base, 1.0)
+        good = detect >= 1.0 - max_detect_loss
+    else:
+        detect = None
+        good = cum_bad <= (1 - target) * n_acc
     table = []
-    for qn in (0.99, 0.95, 0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.3, 0.2):
+    for qn in (1.0, 0.99, 0.95, 0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.3, 0.2, 0.1):
         i = int(qn * (corr.size - 1))
-        table.append((float(corr[order[i]]), float(n_acc[i] / corr.size), float(1 - cum_bad[i] / n_acc[i])))
-    idx = np.flatnonzero(ok)
+        table.append((float(corr[order[i]]), float(n_acc[i] / corr.size), float(1 - cum_bad[i] / n_acc[i]),
+                      float(detect[i]) if detect is not None else None))
+    idx = np.flatnonzero(good)
     gate = float(corr[order[idx[-1]]]) if idx.size else float(corr.max())
     return gate, table
 
@@ -444,9 +468,10 @@ def _cmd_train(args):
 def _cmd_calibrate(net: "ShiftNet", args) -> dict:
     """The gate: on a held-out stack (seed outside the training seeds, images
     of the learned workload's model: 129 px, shifts U[-12,12], 49-shift full
-    grid), the smallest window peak such that among the pairs the window
-    settles at least ``target`` agree with the full-grid search
-    (:func:`calibrate_gate`).  Stored in the weights' meta as ``gate``."""
+    grid, SNR 1 by default), the smallest window peak at which the combined
+    search finds at least (1 - max_detect_loss) of the true common lines the
+    full grid finds (:func:`calibrate_gate` with the true rotations).
+    Stored in the weights' meta as ``gate``."""
     from . import sim
     from .clmatrix import clmatrix_records
 
@@ -455,10 +480,12 @@ def _cmd_calibrate(net: "ShiftNet", args) -> dict:
     full, _ = clmatrix_records(st.pf, 12.0, 0.5, precision="bf16")
     full = {k: v.cpu().numpy() for k, v in full.items()}
     prop = net.propose(st.pf)
-    gate, table = calibrate_gate(st.pf, prop, full, 12.0, 0.5, 1.0, target=args.target)
+    gate, table = calibrate_gate(st.pf, prop, full, 12.0, 0.5, 1.0, target=args.target,
+                                 rots=None if args.no_truth else st.rots, max_detect_loss=args.max_detect_loss)
     _, ws = clmatrix_windowed(st.pf, prop, 12.0, 0.5, 1.0, gate)
-    info = dict(gate=gate, K=args.K, snr=args.snr, seed=args.seed, target=args.target, table=table,
-                window_stats=ws)
+    info = dict(gate=gate, K=args.K, snr=args.snr, seed=args.seed,
+                criterion="agreement >= %g" % args.target if args.no_truth else
+                "detection loss <= %g" % args.max_detect_loss, table=table, window_stats=ws)
     net.meta["gate"] = float(gate)
     net.meta["gate_calibration"] = {k: v for k, v in info.items() if k not in ("table", "window_stats")}
     return info
@@ -482,9 +509,11 @@ def main(argv=None):
     c = sub.add_parser("calibrate", help="pick the window gate on a held-out stack and store it in the weights")
     c.add_argument("--weights", default=WEIGHTS)
     c.add_argument("--K", type=int, default=1000)
-    c.add_argument("--snr", type=float, default=1 / 4)
+    c.add_argument("--snr", type=float, default=1.0)
     c.add_argument("--seed", type=int, default=5101)
-    c.add_argument("--target", type=float, default=0.99)
+    c.add_argument("--target", type=float, default=0.99, help="agreement target (without --no-truth: unused)")
+    c.add_argument("--max-detect-loss", type=float, default=0.005)
+    c.add_argument("--no-truth", action="store_true", help="calibrate on agreement with the full grid instead")
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--K", type=int, default=5000)

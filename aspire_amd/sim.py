@@ -317,24 +317,29 @@ def clmatrix_cheat(rots: np.ndarray, n_theta: int) -> np.ndarray:
     return cl
 
 
-def comparecl(clstack: np.ndarray, ref: np.ndarray, n_theta: int, max_angle: float) -> float:
-    """Fraction of pairs whose detected lines are within max_angle degrees of
-    the true ones, same or flipped orientation (sinograms/comparecl.m:20-52)."""
+def matched_lines(l1, l2, t1, t2, n_theta: int, max_angle: float) -> np.ndarray:
+    """Per pair: both detected lines (l1, l2) within max_angle degrees of the
+    true ones (t1, t2), same or flipped orientation (sinograms/comparecl.m:20-52)."""
     max_angle = max_angle * math.pi / 180
     half = n_theta / 2
     tol = 2 * math.sin(max_angle / 2) + 1.0e-10
     alpha = 2j * math.pi / (2 * half)
-    K = clstack.shape[0]
-    k1, k2 = np.triu_indices(K, 1)
-    t1, t2 = ref[k1, k2], ref[k2, k1]
-    use = t1 != 0
-    l1, l2 = clstack[k1, k2], clstack[k2, k1]
-    e = lambda x: np.exp(alpha * (x - 1))
+    e = lambda x: np.exp(alpha * (np.asarray(x, dtype=np.float64) - 1))
     d1s = np.abs(e(l1) - e(t1))
     d2s = np.abs(e(l2) - e(t2))
     d1f = np.abs(e(l1) * np.exp(1j * math.pi) - e(t1))
     d2f = np.abs(e(l2) * np.exp(1j * math.pi) - e(t2))
-    ok = ((d1s <= tol) & (d2s <= tol)) | ((d1f <= tol) & (d2f <= tol))
+    return ((d1s <= tol) & (d2s <= tol)) | ((d1f <= tol) & (d2f <= tol))
+
+
+def comparecl(clstack: np.ndarray, ref: np.ndarray, n_theta: int, max_angle: float) -> float:
+    """Fraction of pairs whose detected lines are within max_angle degrees of
+    the true ones, same or flipped orientation (sinograms/comparecl.m:20-52)."""
+    K = clstack.shape[0]
+    k1, k2 = np.triu_indices(K, 1)
+    t1, t2 = ref[k1, k2], ref[k2, k1]
+    use = t1 != 0
+    ok = matched_lines(clstack[k1, k2], clstack[k2, k1], t1, t2, n_theta, max_angle)
     return float(ok[use].sum() / max(use.sum(), 1))
 
 

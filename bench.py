@@ -217,10 +217,10 @@ def main():
                                "max_shift=12 step=0.5 (49 shifts) vs window +-1 px (5 shifts) + exhaustive fallback "
                                "below the calibrated gate or off the grid")
         log(f"learned: {learned}")
-        # the same at SNR 1/4, where the full grid finds most true common lines
+        # the same at SNR 1, where the full grid finds most true common lines
         # and its agreement with the window is a meaningful error measure
-        learned["snr_1_4"] = shiftnet.evaluate(net, K=5000, seed=1005, snr=1 / 4, log=lambda *a: None)
-        log(f"learned SNR 1/4: {learned['snr_1_4']}")
+        learned["snr_1"] = shiftnet.evaluate(net, K=5000, seed=1005, snr=1.0, log=lambda *a: None)
+        log(f"learned SNR 1: {learned['snr_1']}")
 
     cpu = None
     if not args.no_cpu:
