@@ -26,7 +26,7 @@ PRECISIONS = {
 # every symbol include/clm.h declares
 EXPORTS = (
     "clm_version", "clm_default_params", "clm_num_pairs", "clm_create",
-    "clm_destroy", "clm_last_error", "clm_clmatrix", "clm_clmatrix_device",
+    "clm_destroy", "clm_last_error", "clm_clmatrix", "clm_clmatrix_device", "clm_clmatrix_device_noalloc",
     "clm_clmatrix_grouped", "clm_autotune", "clm_clmatrix_windowed",
     "clm_shiftnet_nparam", "clm_shiftnet_features", "clm_shiftnet_forward",
     "clm_shiftnet_grad", "clm_shiftnet_adam", "clm_assemble", "clm_shift_equations_csc",
@@ -206,7 +206,7 @@ def load():
     L.clm_clmatrix.restype = ctypes.c_int
     L.clm_clmatrix.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp,
                                ctypes.POINTER(ClmStats)]
-    for name in ("clm_clmatrix_device",):
+    for name in ("clm_clmatrix_device", "clm_clmatrix_device_noalloc"):
         fn = getattr(L, name)
         fn.restype = ctypes.c_int
         fn.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp, ctypes.POINTER(ClmStats)]

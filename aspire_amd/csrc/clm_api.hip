@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -549,7 +550,7 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
                                               (int)pair_slot(k1, k2, K)));
             }
             if (c->tcache.size() >= std::max<size_t>(4, pl.groups.size())) {
-                release(c->tcache.back().tiles);
+                if (!c->arena) release(c->tcache.back().tiles);
                 c->tcache.pop_back();
             }
             c->tcache.insert(c->tcache.begin(), TileCache{});
@@ -1263,6 +1264,65 @@ int autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int sample_pairs
 
 }  // namespace
 
+// Device memory of the allocation-free entry point
+// (clm_clmatrix_device_noalloc): one static array, of which every GPU the
+// library runs on holds its own copy until the process ends.  Host code never
+// passes the array's name to a kernel or a copy: it uses the per-device
+// address from hipGetSymbolAddress, taken once per device at clm_create.
+constexpr size_t NOALLOC_SCRATCH_BYTES = CLM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+hipError_t g_scratch_err[MAX_DEVICES] = {};
+
+// This device's address of `scratch`; the device must be current.
+void* scratch_address(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device] && g_scratch_err[device] == hipSuccess) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && ptr) g_scratch_ptr[device] = ptr;
+        else g_scratch_err[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+    }
+    return g_scratch_ptr[device];
+}
+
+// For the duration of an allocation-free call, the context's per-call device
+// buffers are swapped for empty ones that ensure() carves from the scratch
+// array; the context's own (hipMalloc'd) buffers come back unchanged after.
+struct NoAllocScope {
+    clm_ctx* c;
+    std::vector<StackBufs> sb;
+    std::vector<TileCache> tcache;
+    DevBuf saved[11];
+    DevBuf* shared[11];
+    explicit NoAllocScope(clm_ctx* ctx)
+        : c(ctx), shared{&ctx->table, &ctx->ctr, &ctx->ulist, &ctx->erecs, &ctx->plist, &ctx->tiles2,
+                         &ctx->fblist, &ctx->tw_tiles, &ctx->tw_plist, &ctx->ex_v, &ctx->ex_k} {
+        sb.swap(c->sb);
+        tcache.swap(c->tcache);
+        for (int i = 0; i < 11; i++) {
+            saved[i] = *shared[i];
+            *shared[i] = DevBuf{};
+        }
+        c->arena = (char*)c->scratch;
+        c->arena_cap = c->scratch_cap;
+        c->arena_off = 0;
+    }
+    ~NoAllocScope() {
+        c->arena = nullptr;
+        c->arena_cap = c->arena_off = 0;
+        for (int i = 0; i < 11; i++) *shared[i] = saved[i];
+        sb.swap(c->sb);
+        tcache.swap(c->tcache);
+    }
+};
+}  // namespace
+
 extern "C" {
 
 int clm_version(void) { return CLM_VERSION; }
@@ -1304,6 +1364,10 @@ clm_ctx* clm_create(int device, int* err) {
     }
     if (c->n_cu <= 0) c->n_cu = 256;
     for (auto& e : c->ev) (void)hipEventCreate(&e);
+    // the allocation-free entry point's scratch array on this device (an
+    // unavailable array is reported by that entry point, not here)
+    c->scratch = scratch_address(device);
+    c->scratch_cap = c->scratch ? NOALLOC_SCRATCH_BYTES : 0;
     if (err) *err = CLM_OK;
     return c;
 }
@@ -1351,6 +1415,22 @@ int clm_clmatrix_device(clm_ctx* c, const clm_params* p, const void* d_pf, int32
     if (!d_pf && p->K > 1) return fail(c, CLM_E_ARG, "null pf");
     HIPCHK(c, hipSetDevice(c->device));
     const clm_stack_io io{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
+    return run_job(c, 1, p, &io, false, st);
+}
+
+int clm_clmatrix_device_noalloc(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t* d_cl1, int32_t* d_cl2,
+                                int32_t* d_jidx, double* d_corr, double* d_shift1d, clm_stats* st) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    if (!d_pf && p->K > 1) return fail(c, CLM_E_ARG, "null pf");
+    if (!c->scratch)
+        return fail(c, CLM_E_HIP, "the scratch array of device %d is unavailable (hipGetSymbolAddress: %s)",
+                    c->device,
+                    hipGetErrorString(c->device >= 0 && c->device < MAX_DEVICES ? g_scratch_err[c->device]
+                                                                                : hipErrorInvalidDevice));
+    HIPCHK(c, hipSetDevice(c->device));
+    const clm_stack_io io{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
+    NoAllocScope scope(c);
     return run_job(c, 1, p, &io, false, st);
 }
 

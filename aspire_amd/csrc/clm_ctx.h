@@ -75,6 +75,13 @@ struct clm_ctx {
     // verbose reports and warnings (clm_set_log)
     clm_log_fn log_fn = nullptr;
     void* log_user = nullptr;
+    // allocation-free calls: this device's copy of the library's static scratch
+    // array (from hipGetSymbolAddress at clm_create) and, while such a call
+    // runs, the bump region every device buffer is carved from
+    void* scratch = nullptr;
+    size_t scratch_cap = 0;
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
 };
 
 namespace clmh {
@@ -100,6 +107,19 @@ inline int fail(clm_ctx* c, int code, const char* fmt, ...) {
 inline int ensure(clm_ctx* c, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.n >= bytes) return CLM_OK;
+    if (c->arena) {
+        // allocation-free call: carve from the scratch array, never hipMalloc/hipFree
+        const size_t off = (c->arena_off + 255) & ~(size_t)255;
+        if (off > c->arena_cap || bytes > c->arena_cap - off)
+            return fail(c, CLM_E_UNSUP,
+                        "allocation-free call: the shape needs more device memory than the %zu-byte scratch "
+                        "array (%zu bytes carved, %zu more requested)",
+                        c->arena_cap, off, bytes);
+        b.p = c->arena + off;
+        b.n = bytes;
+        c->arena_off = off + bytes;
+        return CLM_OK;
+    }
     if (b.p) {
         hipError_t e = hipFree(b.p);
         b.p = nullptr;

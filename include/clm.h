@@ -149,6 +149,28 @@ int clm_clmatrix_device(clm_ctx *ctx, const clm_params *p, const void *d_pf,
                         double *d_corr, double *d_shift1d, clm_stats *stats);
 
 /*
+ * Allocation-free form of clm_clmatrix_device.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as clm_clmatrix_device, but every
+ * device buffer the job needs (rays, operand planes, A-stacks, pair records,
+ * lists, counters, tiles) is carved from one static device array of
+ * CLM_NOALLOC_SCRATCH_BYTES that each GPU holds from library load until the
+ * process ends; the call makes no hipMalloc/hipFree.  A shape whose buffers
+ * do not fit returns CLM_E_UNSUP with the sizes in clm_last_error, before
+ * any kernel runs.  Every call on one GPU shares that GPU's array (all
+ * contexts of the process): calls that overlap on one GPU race.  Counters
+ * and lists are cleared by the job itself; other contents persist from the
+ * last call and are overwritten before use.
+ */
+#define CLM_NOALLOC_SCRATCH_BYTES ((size_t)512 << 20)
+int clm_clmatrix_device_noalloc(clm_ctx *ctx, const clm_params *p,
+                                const void *d_pf, int32_t *d_cl1,
+                                int32_t *d_cl2, int32_t *d_jidx,
+                                double *d_corr, double *d_shift1d,
+                                clm_stats *stats);
+
+/*
  * cryo_clmatrix with map_filter_radius > 0 (its sixth argument,
  * cryo_clmatrix_cpu.m:24-29,383-385): every shift's L x 2L correlation map
  * is filtered by cryo_average_clmap (sinograms/cryo_average_clmap.m:20-48,
