@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libclm.so")
 CLM_FP32, CLM_BF16_RESCORE, CLM_FP16_RESCORE, CLM_FP64_REF = 0, 1, 2, 3
 CLM_C128, CLM_C64 = 0, 1
 CLM_R64, CLM_R32 = 0, 1
+CLM_PFT_EXACT, CLM_PFT_NUFFT = 0, 1
 
 PRECISIONS = {
     "fp32": CLM_FP32,
@@ -125,6 +126,7 @@ class ClmPftParams(ctypes.Structure):
         ("n_theta", ctypes.c_int32),
         ("img_dtype", ctypes.c_int32),
         ("pf_dtype", ctypes.c_int32),
+        ("method", ctypes.c_int32),
     ]
 
 
@@ -134,6 +136,8 @@ class ClmPftStats(ctypes.Structure):
         ("ms_stage", ctypes.c_float),
         ("ms_pft", ctypes.c_float),
         ("chunks", ctypes.c_int32),
+        ("ms_fft", ctypes.c_float),
+        ("grid_side", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -26,6 +26,7 @@ UNITS = [(os.path.join(CSRC, "clm_api.hip"), []), (os.path.join(CSRC, "shiftnet.
     (os.path.join(CSRC, "pair_variants.hip"), [f"-DCLM_VARIANT_OP={op}", f"-DCLM_VARIANT_SYM={sym}"])
     for op in ("bf16", "f16", "f32") for sym in (1, 0)]
 ARCH = "gfx950"
+ROCM_LIB = "/opt/rocm/lib"
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -71,7 +72,9 @@ def build(verbose: bool = False) -> str:
         if failed:
             raise RuntimeError(f"hipcc failed with exit code {failed}")
         tmp = LIB + ".tmp"
-        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+        # hipFFT: the grid transforms of cryo_pft's NUFFT method
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+               f"-L{ROCM_LIB}", "-lhipfft", f"-Wl,-rpath,{ROCM_LIB}"]
         if verbose:
             print(" ".join(cmd), flush=True)
         res = subprocess.run(cmd, capture_output=True, text=True)

@@ -1392,7 +1392,9 @@ void clm_destroy(clm_ctx* c) {
     DevBuf* wb[] = {&c->w_cl1, &c->w_cl2, &c->w_j, &c->w_corr, &c->w_shift, &c->w_lsh, &c->w_flag, &c->w_rows,
                     &c->w_list, &c->w_plist, &c->w_tiles, &c->w_counts, &c->w_shiftv};
     for (DevBuf* b : wb) release(*b);
+    pft_release(c);
     DevBuf* pb[] = {&c->pft_ax, &c->pft_ay, &c->pft_xt, &c->pft_img, &c->pft_out,
+                    &c->pft_grid, &c->pft_base, &c->pft_wt, &c->pft_dec,
                     &c->ls_pack, &c->ls_u, &c->ls_part, &c->ls_vec, &c->ls_tab, &c->ls_state, &c->ls_tiles,
                     &c->vt_m, &c->vt_mt, &c->vt_pk, &c->vt_cs,
                     &c->s3_rs, &c->s3_vs, &c->s3_ss, &c->s3_tiles, &c->s3_thr, &c->s3_hist};

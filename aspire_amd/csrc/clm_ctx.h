@@ -52,8 +52,14 @@ struct clm_ctx {
     clmh::DevBuf ex_v, ex_k;
     // shift network: activations and gradients of the last batch
     clmh::DevBuf sn_h1, sn_h2, sn_z, sn_y, sn_dy, sn_dz, sn_dz2, sn_dz1;
-    // polar Fourier transform: per-axis tables, staged images, host-form buffers
+    // polar Fourier transform: per-axis tables, staged images, host-form buffers;
+    // the NUFFT method's oversampled grids, windows, kernel weights and
+    // deconvolution factors, and its hipFFT plans (opaque handles) with their
+    // grid side and batch
     clmh::DevBuf pft_ax, pft_ay, pft_xt, pft_img, pft_out;
+    clmh::DevBuf pft_grid, pft_base, pft_wt, pft_dec;
+    void* fft_plan[2] = {nullptr, nullptr};
+    int fft_side[2] = {0, 0}, fft_batch[2] = {0, 0};
     // shift least squares: packed lines, u, per-tile sums, 2K vectors and the tiles' partial
     // A'u sums, tables, scalar state, pair tiles
     clmh::DevBuf ls_pack, ls_u, ls_part, ls_vec, ls_tab, ls_state, ls_tiles;
@@ -85,6 +91,9 @@ struct clm_ctx {
 };
 
 namespace clmh {
+
+// Releases the hipFFT plans of the polar Fourier transform (pft.hip).
+void pft_release(clm_ctx* c);
 
 inline int fail(clm_ctx* c, int code, const char* fmt, ...) {
     char buf[512];

@@ -139,18 +139,22 @@ __device__ inline bool better64(double v, long long k, double bv, long long bk) 
     return v > bv || (v == bv && k < bk);
 }
 
-// C(l1, l2p) at shift j in FP64, the oracle's arithmetic and order.
-__device__ inline double entry64(const double2* P1, const double2* P2, const double2* ph, int R, bool flip) {
-    double acc = 0.0;
-    for (int r = 0; r < R; r++) {
-        const double pr = P1[r].x, pi = P1[r].y, qr = ph[r].x, qi = ph[r].y;
-        double xr, xi;
-        if (!flip) { xr = pr * qr - pi * qi; xi = pr * qi + pi * qr; }
-        else       { xr = pr * qr + pi * qi; xi = pr * qi - pi * qr; }
-        const double t = xr * P2[r].x, u = xi * P2[r].y;
-        acc = acc + (t + u);
-    }
-    return 2.0 * acc;
+// Term r of C(l1, l2p) at shift j in FP64, the oracle's arithmetic: the row
+// a = P1(r).*ph_j(r) (conj(P1) for the C2 half, flip), then a.re*b.re +
+// a.im*b.im.
+__device__ inline double term64(double2 p, double2 q, double2 b, bool flip) {
+    double xr, xi;
+    if (!flip) { xr = p.x * q.x - p.y * q.y; xi = p.x * q.y + p.y * q.x; }
+    else       { xr = p.x * q.x + p.y * q.y; xi = p.x * q.y - p.y * q.x; }
+    const double t = xr * b.x, u = xi * b.y;
+    return t + u;
+}
+
+__device__ inline double readlane64(double x, int l) {
+    const unsigned long long u = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Stack of flat pair index `item` (stacks sorted by pair_base).
@@ -182,14 +186,27 @@ __device__ inline void cand_rc(const Geom& g, int mode, const PairRec& rec, cons
     if (row >= g.NROWS || col >= g.L) row = -1;
 }
 
-__device__ inline double eval64(const StackDev& sd, int k1, int k2, long long key) {
+// C at tie key `key` of pair (k1, k2) in FP64, bit-identical to the oracle:
+// lane r forms term r (r = lane, lane + 64, ...) and the terms are added in
+// order r = 0..R-1, acc = acc + term, as the oracle adds them.  Called by the
+// whole wave with a wave-uniform key; the result is the same in every lane.
+__device__ inline double eval64(const StackDev& sd, int k1, int k2, long long key, int lane) {
     const Geom& g = sd.g;
     const int l1 = (int)(key % g.L);
     const int l2p = (int)((key / g.L) % (2 * g.L));
     const int j = (int)(key / ((long long)g.L * 2 * g.L));
     const double2* P1 = sd.P64 + ((size_t)k1 * g.L + l1) * g.R;
     const double2* P2 = sd.P64 + ((size_t)k2 * g.L + (l2p % g.L)) * g.R;
-    return entry64(P1, P2, sd.ph64 + (size_t)j * g.R, g.R, l2p >= g.L);
+    const double2* ph = sd.ph64 + (size_t)j * g.R;
+    const bool flip = l2p >= g.L;
+    double acc = 0.0;
+    for (int c0 = 0; c0 < g.R; c0 += 64) {
+        const int r = c0 + lane;
+        const double tr = r < g.R ? term64(P1[r], ph[r], P2[r], flip) : 0.0;
+        const int n = min(64, g.R - c0);
+        for (int q = 0; q < n; q++) acc = acc + readlane64(tr, q);
+    }
+    return 2.0 * acc;
 }
 
 __device__ inline void write_result(const StackDev& sd, int slot, double v, long long key) {
@@ -216,9 +233,9 @@ constexpr int RS_MAXV = 2 * MAXE;  // FP32 values kept per pair (2 per row)
 
 // One wave per pair.  FP32 re-score with 4 lanes per candidate row (16 rows
 // per step, the row's lanes reading adjacent 16-byte pieces), values kept in
-// LDS; then FP64 re-score
-// of every candidate within tau of the FP32 best, one lane per candidate with
-// the oracle's sequential arithmetic.
+// LDS; then FP64 re-score of every candidate within tau of the FP32 best,
+// the whole wave on one candidate (one lane per term, the terms added in
+// the oracle's order).
 __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
     __shared__ float s_val[4][RS_MAXV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -335,7 +352,9 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
         long long fk = bk;
         if (sd.fp64 && bv > -INFINITY) {
             __builtin_amdgcn_wave_barrier();
-            // ---- FP64 re-score of every candidate within tau of the FP32 best
+            // ---- FP64 re-score of every candidate within tau of the FP32
+            // best, one candidate at a time across the wave (the best by
+            // value, then tie key, does not depend on the visiting order)
             const float lim = bv - sd.tau;
             double b64 = -INFINITY;
             long long k64 = 0x7fffffffffffffffLL;
@@ -343,24 +362,20 @@ __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
             const int nv = min(2 * nrows, RS_MAXV);
             for (int base = 0; base < nv; base += 64) {
                 const int iv = base + lane;
-                if (iv < nv && sval[iv] >= lim) {
+                unsigned long long mask = __ballot(iv < nv && sval[iv] >= lim);
+                while (mask) {
+                    const int ic = base + __builtin_ctzll(mask);
+                    mask &= mask - 1;
                     int row, col;
-                    cand_rc(g, cmode, rec, er, iv >> 1, row, col);
+                    cand_rc(g, cmode, rec, er, ic >> 1, row, col);
                     const int e = row / g.L, l1 = row - e * g.L;
                     long long key;
-                    if (g.sym) key = (iv & 1) ? tie_key(g.S - 1 - e, g.L + col, l1, g.L) : tie_key(e, col, l1, g.L);
+                    if (g.sym) key = (ic & 1) ? tie_key(g.S - 1 - e, g.L + col, l1, g.L) : tie_key(e, col, l1, g.L);
                     else key = e < g.S ? tie_key(e, col, l1, g.L) : tie_key(e - g.S, g.L + col, l1, g.L);
-                    const double v = eval64(sd, k1, k2, key);
+                    const double v = eval64(sd, k1, k2, key, lane);
                     n64++;
                     if (!isnan(v) && better64(v, key, b64, k64)) { b64 = v; k64 = key; }
                 }
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const double ov = __shfl_xor(b64, o, 64);
-                const long long ok = __shfl_xor(k64, o, 64);
-                if (better64(ov, ok, b64, k64)) { b64 = ov; k64 = ok; }
-                n64 += __shfl_xor(n64, o, 64);
             }
             c_64 += (unsigned long long)n64;
             fv = b64;

@@ -21,9 +21,27 @@
 // and a per-point rotation, and each lane forms its operand with complex
 // products in registers.  Images are first staged pixel-major (pixel rows of Kp images,
 // FP64) so the B fragments load as 128-byte rows.
+//
+// CLM_PFT_NUFFT: the same sums by a type-2 non-uniform FFT, the method the
+// reference's nufft2 uses when a NUFFT library is installed
+// (fourier/nufft/nufft_transform.m:108-150, finufft with epsilon 1e-15 by
+// default, nufft_initialize.m:29).  Own implementation of the published
+// exponential-of-semicircle method (Barnett, Magland and af Klinteberg, SIAM
+// J. Sci. Comput. 41(5), 2019): kernel phi(z) = exp(beta (sqrt(1 - z^2) - 1))
+// of width NW = 16 grid points, beta = 2.30 NW, on an oversampled grid of
+// side N >= 2n (2,3,5-smooth).  Per image:
+//   a(x mod N, y mod N) = p(u, v) / (Phi(x) Phi(y)),  x = g(u), y = g(v)
+//   b = the unnormalized backward 2-D FFT of a (sum a e^{+2 pi i l x / N})
+//   pf(m) = sum_{t, r < NW} phi(2 (sx - lx - t)/NW) phi(2 (sy - ly - r)/NW) b(lx + t, ly + r)
+// with sx = fx(m) N / (2 pi), lx = ceil(sx - NW/2) (sy, ly alike, indices mod
+// N) and Phi(x) = int phi(2s/NW) cos(2 pi s x / N) ds over |s| <= NW/2, the
+// kernel's Fourier transform (Gauss-Legendre quadrature, on the host).  The
+// result differs from the exact sums by about 1e-14 of max |pf|.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <cmath>
+#include <vector>
 
 #include "clm_ctx.h"
 
@@ -179,6 +197,138 @@ __global__ __launch_bounds__(256, 2) void pft_kernel(Geo g, const double* __rest
         }
 }
 
+// ------------------------------------------------------------ type-2 NUFFT
+constexpr int NW = 16;                  // kernel width, grid points per axis
+constexpr double NBETA = 2.30 * NW;     // kernel shape for oversampling 2
+constexpr int NCHUNK = 2048;            // images per batch of FFTs (at most)
+constexpr int NIPW = 4;                 // images per interpolation workgroup
+
+struct NGeo {
+    int n, n_r, n_theta, M;  // image side, rays, polar points M = n_r * n_theta/2
+    int N;                   // oversampled grid side
+    int x0;                  // g(0) = ceil(-n/2): pixel u sits at x = x0 + u
+    int Kc;                  // images in this chunk
+};
+
+__device__ inline double es_kernel(double z) {
+    return fabs(z) <= 1.0 ? exp(NBETA * (sqrt(1.0 - z * z) - 1.0)) : 0.0;
+}
+
+// Per polar point m: the first grid index of its window on each axis and
+// the NW kernel weights along x, then along y.  Frequencies as in
+// tables_kernel (cryo_pft.m:49-58).
+__global__ void nweights_kernel(NGeo g, int2* base, double* wt) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.M) return;
+    const int k = m % g.n_r, j = m / g.n_r;
+    const double omega0 = 2.0 * M_PI / (2.0 * g.n_r - 1.0);
+    const double dtheta = 2.0 * M_PI / g.n_theta;
+    const double fx = (k * omega0) * sin(j * dtheta), fy = (k * omega0) * cos(j * dtheta);
+    const double sx = fx * (g.N / (2.0 * M_PI)), sy = fy * (g.N / (2.0 * M_PI));
+    const int lx = (int)ceil(sx - 0.5 * NW), ly = (int)ceil(sy - 0.5 * NW);
+    base[m] = make_int2(lx, ly);
+    double* w = wt + (size_t)m * 2 * NW;
+    for (int t = 0; t < NW; t++) {
+        w[t] = es_kernel(2.0 * (sx - (lx + t)) / NW);
+        w[NW + t] = es_kernel(2.0 * (sy - (ly + t)) / NW);
+    }
+}
+
+// Grid of the chunk's images, a[i][y mod N][x mod N] (x contiguous):
+// p(u, v, k0 + i) / (dec[u] dec[v]) where a pixel sits, zero elsewhere.
+template <typename T>
+__global__ void deconv_kernel(NGeo g, const T* __restrict__ img, long long k0, const double* __restrict__ dec,
+                              double2* __restrict__ a) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long NN = (long long)g.N * g.N;
+    if (q >= NN * g.Kc) return;
+    const int i = (int)(q / NN);
+    const int r = (int)(q - (long long)i * NN);
+    const int ly = r / g.N, lx = r - ly * g.N;
+    const int xmax = g.x0 + g.n - 1;  // pixel offsets x0..xmax; N >= 2n keeps x and x - N apart
+    const int x = lx <= xmax ? lx : lx - g.N, y = ly <= xmax ? ly : ly - g.N;
+    double val = 0.0;
+    if (x >= g.x0 && y >= g.x0) {
+        const int u = x - g.x0, v = y - g.x0;
+        val = (double)img[((size_t)(k0 + i) * g.n + v) * g.n + u] / (dec[u] * dec[v]);
+    }
+    a[q] = make_double2(val, 0.0);
+}
+
+// Sum over the 16 lanes of each row (quad_perm [1,0,3,2], [2,3,0,1],
+// row_ror 4, row_ror 8): every lane of the row gets it.  All lanes active.
+template <int CTRL>
+__device__ inline double dpp64(double x) {
+    const unsigned long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffu), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double row16_sum(double x) {
+    x += dpp64<0xb1>(x);
+    x += dpp64<0x4e>(x);
+    x += dpp64<0x124>(x);
+    x += dpp64<0x128>(x);
+    return x;
+}
+
+__device__ inline void put_pf(double2* o, double re, double im) { *o = make_double2(re, im); }
+__device__ inline void put_pf(float2* o, double re, double im) { *o = make_float2((float)re, (float)im); }
+
+// pf of the chunk's images at the polar points.  Workgroup: 16 points (4
+// per wave, 16 lanes each; lane t = x offset of the window) x NIPW images;
+// each lane walks the NW window rows in order, then the point's 16 lanes are
+// summed.  pf(k, j) and its conjugate pf(k, j + n_theta/2) (cryo_pft.m:75).
+template <typename OutT>
+__global__ __launch_bounds__(256) void interp_kernel(NGeo g, const double2* __restrict__ b,
+                                                     const int2* __restrict__ base, const double* __restrict__ wt,
+                                                     OutT* __restrict__ pf, long long k0) {
+    __shared__ double s_wy[16][NW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane & 15, ps = wave * 4 + (lane >> 4);
+    const int m = blockIdx.x * 16 + ps;
+    const bool act = m < g.M;
+    int cx = 0, ry0 = 0;
+    double wx = 0.0;
+    if (act) {
+        const int2 bs = base[m];
+        wx = wt[(size_t)m * 2 * NW + t];
+        s_wy[ps][t] = wt[(size_t)m * 2 * NW + NW + t];
+        cx = (bs.x + t) % g.N;
+        if (cx < 0) cx += g.N;
+        ry0 = bs.y % g.N;
+        if (ry0 < 0) ry0 += g.N;
+    }
+    __syncthreads();
+    const int L = g.n_theta / 2;
+    const int k = act ? m % g.n_r : 0, j = act ? m / g.n_r : 0;
+    const long long NN = (long long)g.N * g.N;
+    for (int ii = 0; ii < NIPW; ii++) {
+        const int i = blockIdx.y * NIPW + ii;
+        if (i >= g.Kc) break;  // the same for the whole workgroup
+        const double2* bi = b + (size_t)i * NN;
+        double re = 0.0, im = 0.0;
+        if (act) {
+            int ry = ry0;
+#pragma unroll 4
+            for (int r = 0; r < NW; r++) {
+                const double w = s_wy[ps][r] * wx;
+                const double2 v = bi[(size_t)ry * g.N + cx];
+                re += w * v.x;
+                im += w * v.y;
+                ry = ry + 1 == g.N ? 0 : ry + 1;
+            }
+        }
+        re = row16_sum(re);
+        im = row16_sum(im);
+        if (act && t == 0) {
+            const size_t col = (size_t)(k0 + i) * g.n_theta;
+            put_pf(pf + (col + j) * g.n_r + k, re, im);
+            put_pf(pf + (col + j + L) * g.n_r + k, re, -im);
+        }
+    }
+}
+
 }  // namespace pft
 
 using namespace clmh;
@@ -196,6 +346,10 @@ int pft_check(clm_ctx* c, const clm_pft_params* p) {
     if (p->img_dtype != CLM_R64 && p->img_dtype != CLM_R32)
         return fail(c, CLM_E_ARG, "unknown image dtype %d", p->img_dtype);
     if (p->pf_dtype != CLM_C128 && p->pf_dtype != CLM_C64) return fail(c, CLM_E_ARG, "unknown pf dtype %d", p->pf_dtype);
+    if (p->method != CLM_PFT_EXACT && p->method != CLM_PFT_NUFFT)
+        return fail(c, CLM_E_ARG, "unknown pft method %d", p->method);
+    if (p->method == CLM_PFT_NUFFT && p->n > 2048)
+        return fail(c, CLM_E_UNSUP, "image side n = %d: the NUFFT method takes n <= 2048", p->n);
     return CLM_OK;
 }
 
@@ -269,7 +423,172 @@ int pft_run(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, 
     return CLM_OK;
 }
 
+// ------------------------------------------------------------ type-2 NUFFT
+// Gauss-Legendre rule of q nodes on [-1, 1]: roots of P_q by Newton's method
+// (P_q from the three-term recurrence), weights 2 / ((1 - z^2) P_q'(z)^2).
+void gauss_legendre(int q, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(q, 0.0);
+    w.assign(q, 0.0);
+    for (int i = 0; i < (q + 1) / 2; i++) {
+        double z = std::cos(M_PI * (i + 0.75) / (q + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; it++) {
+            double p1 = 1.0, p0 = 0.0;
+            for (int jj = 1; jj <= q; jj++) {
+                const double pm = p0;
+                p0 = p1;
+                p1 = ((2.0 * jj - 1.0) * z * p0 - (jj - 1.0) * pm) / jj;
+            }
+            dp = q * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[i] = -z;
+        x[q - 1 - i] = z;
+        w[i] = w[q - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+// Smallest 2^a 3^b 5^c >= n.
+int smooth_at_least(int n) {
+    for (int m = n;; m++) {
+        int r = m;
+        for (int f : {2, 3, 5})
+            while (r % f == 0) r /= f;
+        if (r == 1) return m;
+    }
+}
+
+hipfftHandle fft_plan(clm_ctx* c, int N, int batch, int* rc) {
+    for (int s = 0; s < 2; s++)
+        if (c->fft_plan[s] && c->fft_side[s] == N && c->fft_batch[s] == batch) return (hipfftHandle)c->fft_plan[s];
+    // replace the older of the two (slot 1), keeping slot 0 the most recent
+    if (c->fft_plan[1]) (void)hipfftDestroy((hipfftHandle)c->fft_plan[1]);
+    c->fft_plan[1] = c->fft_plan[0];
+    c->fft_side[1] = c->fft_side[0];
+    c->fft_batch[1] = c->fft_batch[0];
+    c->fft_plan[0] = nullptr;
+    hipfftHandle h = nullptr;
+    int dims[2] = {N, N};
+    hipfftResult r = hipfftPlanMany(&h, 2, dims, nullptr, 1, N * N, nullptr, 1, N * N, HIPFFT_Z2Z, batch);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, c->stream);
+    if (r != HIPFFT_SUCCESS) {
+        if (h) (void)hipfftDestroy(h);
+        *rc = fail(c, CLM_E_HIP, "hipFFT plan of %d x %d x %d failed (hipfftResult %d)", N, N, batch, (int)r);
+        return nullptr;
+    }
+    c->fft_plan[0] = (void*)h;
+    c->fft_side[0] = N;
+    c->fft_batch[0] = batch;
+    return h;
+}
+
+int nufft_run(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, clm_pft_stats* st) {
+    using namespace pft;
+    NGeo g{};
+    g.n = p->n;
+    g.n_r = p->n_r;
+    g.n_theta = p->n_theta;
+    g.M = p->n_r * (p->n_theta / 2);
+    g.N = smooth_at_least(std::max(2 * p->n, 2 * NW));
+    g.x0 = (int)std::ceil(-0.5 * p->n);
+    const int K = p->K;
+    const size_t NN = (size_t)g.N * g.N;
+    // images per batch: at most NCHUNK, and grids of at most 4 GiB
+    const int kc = (int)std::max<long long>(1, std::min<long long>({(long long)K, (long long)NCHUNK,
+                                                                     (4LL << 30) / (long long)(NN * sizeof(double2))}));
+    int rc;
+    if ((rc = ensure(c, c->pft_grid, (size_t)kc * NN * sizeof(double2))) ||
+        (rc = ensure(c, c->pft_base, (size_t)g.M * sizeof(int2))) ||
+        (rc = ensure(c, c->pft_wt, (size_t)g.M * 2 * NW * sizeof(double))) ||
+        (rc = ensure(c, c->pft_dec, (size_t)g.n * sizeof(double))))
+        return rc;
+    // deconvolution factors Phi(x0 + u), u < n
+    std::vector<double> gx, gw, dec(g.n);
+    gauss_legendre(200, gx, gw);
+    for (int u = 0; u < g.n; u++) {
+        const double x = g.x0 + u;
+        double acc = 0.0;
+        for (size_t q = 0; q < gx.size(); q++)
+            acc += gw[q] * std::exp(NBETA * (std::sqrt(1.0 - gx[q] * gx[q]) - 1.0)) *
+                   std::cos(M_PI * gx[q] * NW * x / g.N);
+        dec[u] = 0.5 * NW * acc;
+    }
+    hipEvent_t* ev = c->ev;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pft_dec.p, dec.data(), dec.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    nweights_kernel<<<(g.M + 255) / 256, 256, 0, c->stream>>>(g, (int2*)c->pft_base.p, (double*)c->pft_wt.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    float ms_stage = 0.f, ms_fft = 0.f, ms_pft = 0.f;
+    int chunks = 0;
+    double2* grid = (double2*)c->pft_grid.p;
+    for (long long k0 = 0; k0 < K; k0 += kc) {
+        g.Kc = (int)((K - k0) < kc ? (K - k0) : kc);
+        hipfftHandle plan = fft_plan(c, g.N, g.Kc, &rc);
+        if (!plan) return rc;
+        HIPCHK(c, hipEventRecord(ev[2], c->stream));
+        const long long nq = (long long)NN * g.Kc;
+        if (p->img_dtype == CLM_R64)
+            deconv_kernel<double><<<(unsigned)((nq + 255) / 256), 256, 0, c->stream>>>(
+                g, (const double*)d_img, k0, (const double*)c->pft_dec.p, grid);
+        else
+            deconv_kernel<float><<<(unsigned)((nq + 255) / 256), 256, 0, c->stream>>>(
+                g, (const float*)d_img, k0, (const double*)c->pft_dec.p, grid);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[3], c->stream));
+        const hipfftResult fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex*)grid, (hipfftDoubleComplex*)grid,
+                                              HIPFFT_BACKWARD);
+        if (fr != HIPFFT_SUCCESS) return fail(c, CLM_E_HIP, "hipfftExecZ2Z failed (hipfftResult %d)", (int)fr);
+        HIPCHK(c, hipEventRecord(ev[4], c->stream));
+        const dim3 igrid((g.M + 15) / 16, (g.Kc + NIPW - 1) / NIPW);
+        if (p->pf_dtype == CLM_C128)
+            interp_kernel<double2><<<igrid, 256, 0, c->stream>>>(g, grid, (const int2*)c->pft_base.p,
+                                                                 (const double*)c->pft_wt.p, (double2*)d_pf, k0);
+        else
+            interp_kernel<float2><<<igrid, 256, 0, c->stream>>>(g, grid, (const int2*)c->pft_base.p,
+                                                                (const double*)c->pft_wt.p, (float2*)d_pf, k0);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev[5], c->stream));
+        HIPCHK(c, hipEventSynchronize(ev[5]));
+        float a = 0.f, b = 0.f, d = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, ev[2], ev[3]));
+        HIPCHK(c, hipEventElapsedTime(&b, ev[3], ev[4]));
+        HIPCHK(c, hipEventElapsedTime(&d, ev[4], ev[5]));
+        ms_stage += a;
+        ms_fft += b;
+        ms_pft += d;
+        chunks++;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // dec is released on return
+    if (st) {
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[1]));
+        st->ms_tables = t;
+        st->ms_stage = ms_stage;
+        st->ms_fft = ms_fft;
+        st->ms_pft = ms_pft;
+        st->chunks = chunks;
+        st->grid_side = g.N;
+    }
+    return CLM_OK;
+}
+
+int pft_dispatch(clm_ctx* c, const clm_pft_params* p, const void* d_img, void* d_pf, clm_pft_stats* st) {
+    return p->method == CLM_PFT_NUFFT ? nufft_run(c, p, d_img, d_pf, st) : pft_run(c, p, d_img, d_pf, st);
+}
+
 }  // namespace
+
+namespace clmh {
+void pft_release(clm_ctx* c) {
+    for (int s = 0; s < 2; s++) {
+        if (c->fft_plan[s]) (void)hipfftDestroy((hipfftHandle)c->fft_plan[s]);
+        c->fft_plan[s] = nullptr;
+        c->fft_side[s] = c->fft_batch[s] = 0;
+    }
+}
+}  // namespace clmh
 
 extern "C" {
 
@@ -296,7 +615,7 @@ int clm_pft_device(clm_ctx* c, const clm_pft_params* p, const void* d_img, void*
     if (p->K == 0) return CLM_OK;
     if (!d_img || !d_pf) return fail(c, CLM_E_ARG, "null image or pf pointer");
     HIPCHK(c, hipSetDevice(c->device));
-    return pft_run(c, p, d_img, d_pf, st);
+    return pft_dispatch(c, p, d_img, d_pf, st);
 }
 
 int clm_pft(clm_ctx* c, const clm_pft_params* p, const void* img, void* pf, clm_pft_stats* st) {
@@ -311,7 +630,7 @@ int clm_pft(clm_ctx* c, const clm_pft_params* p, const void* img, void* pf, clm_
     const size_t npf = (size_t)p->n_r * p->n_theta * p->K * (p->pf_dtype == CLM_C128 ? 16 : 8);
     if ((rc = ensure(c, c->pft_img, nimg)) || (rc = ensure(c, c->pft_out, npf))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->pft_img.p, img, nimg, hipMemcpyHostToDevice, c->stream));
-    if ((rc = pft_run(c, p, c->pft_img.p, c->pft_out.p, st))) return rc;
+    if ((rc = pft_dispatch(c, p, c->pft_img.p, c->pft_out.p, st))) return rc;
     HIPCHK(c, hipMemcpyAsync(pf, c->pft_out.p, npf, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CLM_OK;

@@ -36,9 +36,11 @@ namespace lsq {
 struct State {
     double alpha, beta, bnorm, phibar, rhobar, anorm, res2, rnorm, arnorm;
     double damp, tol;
-    double t1, t2, ia;  // this iteration's update of x, w and v
+    double t1, t2;      // the last scalar step's update of x and w
+    double vscale;      // the current v is the stored v times vscale
     int itn, maxit, done, flag;
-    int upd;            // 1: this iteration's scalar step ran, apply its update
+    int upd;            // 1: the last scalar step's update of x and w is pending
+    int vset;           // 1: the first direction v was formed (b != 0)
 };
 
 __device__ inline long long first(long long k1, long long K) { return k1 * K - k1 * (k1 + 1) / 2; }
@@ -87,6 +89,27 @@ __device__ inline double block_sum(double v, double* sh) {
     return r;
 }
 
+// Sum of p[0..n) in one fixed order whatever the block size NT >= 256:
+// thread t < 256 adds p[t], p[t + 256], ... in order, then a 256-wide tree.
+// Every block that calls it gets the same bits.
+template <int NT>
+__device__ inline double fixed_sum(const double* p, int n, double* sh) {
+    const int t = threadIdx.x;
+    double s = 0.0;
+    if (t < 256)
+        for (int i = t; i < n; i += 256) s += p[i];
+    if (t < 256) sh[t] = s;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
 // Pair tiles: rows k1 in [TR*kb, TR*kb + TR), columns k2 in [TC*cb, TC*cb + TC);
 // a tile exists when it holds a pair k1 < k2 < K.
 constexpr int TR = 64, TC = 256;
@@ -96,11 +119,33 @@ __host__ __device__ inline bool tile_exists(int K, int kb, int cb) {
     return TR * kb < kmax;
 }
 
-// Fixed-order sum over the 64 lanes (xor butterfly); every lane gets it.
+// x from the lane that DPP control CTRL names (both 32-bit halves moved
+// alike); every lane of the wave must be active.
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+    const unsigned long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffu), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ inline double readlane_d(double x, int l) {
+    const unsigned long long u = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes in a fixed order, the same in every lane: DPP adds
+// within each row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_ror 4,
+// row_ror 8), then the four row sums in row order.  Every lane of the wave
+// must be active.
 __device__ inline double wave_sum(double x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
+    x += dpp_d<0xb1>(x);
+    x += dpp_d<0x4e>(x);
+    x += dpp_d<0x124>(x);
+    x += dpp_d<0x128>(x);
+    return ((readlane_d(x, 0) + readlane_d(x, 16)) + readlane_d(x, 32)) + readlane_d(x, 48);
 }
 
 // One pass over tile t's pairs (row-major, pair e = first(k1) + k2 - k1 - 1):
@@ -128,10 +173,11 @@ __global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __r
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int k2 = TC * cb + t;
     const double ca = INIT ? 0.0 : S->alpha / S->beta;
+    const double vs = INIT ? 0.0 : S->vscale;  // stored v times vscale is the current v
     double va0 = 0.0, va1 = 0.0;
     if (!INIT && k2 < K) {
-        va0 = v[2 * k2];
-        va1 = v[2 * k2 + 1];
+        va0 = v[2 * k2] * vs;
+        va1 = v[2 * k2 + 1] * vs;
     }
     double c0 = 0.0, c1 = 0.0, sq = 0.0;
     const unsigned z = (unsigned)(2 * L + 1);
@@ -156,7 +202,8 @@ __global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __r
             const double* B = tb + 3 * (pk >> 16);
             const double sg = B[2];
             if (!INIT) {
-                const double av = sg * A.x * v[2 * k1] + sg * A.y * v[2 * k1 + 1] + B[0] * va0 + B[1] * va1;
+                const double av = sg * A.x * (v[2 * k1] * vs) + sg * A.y * (v[2 * k1 + 1] * vs) + B[0] * va0 +
+                                  B[1] * va1;
                 un = av - ca * u[e];
             }
             u[e] = un;
@@ -188,70 +235,93 @@ __global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __r
     if (t == 0) normp[blockIdx.x] = ((s_sq[0] + s_sq[1]) + s_sq[2]) + s_sq[3];
 }
 
-// beta = sqrt(sum of the tiles' squares, in tile order); first call (init):
-// bnorm = beta; later: the running Frobenius estimate of A.
-__global__ __launch_bounds__(1024) void beta_kernel(int n, const double* part, State* S, int init) {
+// Before the first iteration: beta = bnorm = |b| from the first pass's
+// tiles' sums of squares.
+__global__ __launch_bounds__(1024) void beta_kernel(int n, const double* part, State* S) {
     __shared__ double sh[1024];
     if (S->done) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) s += part[i];
-    s = block_sum<1024>(s, sh);
+    const double beta = sqrt(fixed_sum<1024>(part, n, sh));
     if (threadIdx.x == 0) {
-        const double beta = sqrt(s);
         S->beta = beta;
-        if (init) {
-            S->bnorm = beta;
-            if (beta == 0.0) {  // b = 0: x = 0 is the solution
-                S->done = 1;
-                S->flag = 0;
-            }
-        } else if (beta > 0.0) {
-            S->anorm = sqrt(S->anorm * S->anorm + S->alpha * S->alpha + beta * beta + S->damp * S->damp);
+        S->bnorm = beta;
+        if (beta == 0.0) {  // b = 0: x = 0 is the solution
+            S->done = 1;
+            S->flag = 0;
         }
     }
 }
 
-// (A'u)[i] for entry i of the 2K-vector: the row parts of the tiles in
-// image i/2's row block, then the column parts of the tiles in its column
-// block, each in block order.
-__device__ inline double atu_value(int K, const double* rowp, const double* colp, int i) {
+// Group g's share (g = 0..3) of (A'u)[i], entry i of the 2K-vector: the
+// row parts of the tiles in image i/2's row block, then the column parts of
+// the tiles in its column block, each in block order over the g-th quarter
+// of the blocks.  (A'u)[i] = ((g0 + g1) + g2) + g3.
+__device__ inline double atu_part(int K, const double* rowp, const double* colp, int i, int g) {
     const int k = i >> 1, n2 = 2 * K;
     const int kb = k / TR, cbk = k / TC;
     const int ncb = (K + TC - 1) / TC, nkb = (K + TR - 1) / TR;
     double t = 0.0;
-    for (int cb = 0; cb < ncb; cb++)
+    for (int cb = g * ncb / 4; cb < (g + 1) * ncb / 4; cb++)
         if (tile_exists(K, kb, cb)) t += rowp[(size_t)cb * n2 + i];
-    for (int b = 0; b < nkb; b++)
+    for (int b = g * nkb / 4; b < (g + 1) * nkb / 4; b++)
         if (tile_exists(K, b, cbk)) t += colp[(size_t)b * n2 + i];
     return t;
 }
 
-// First direction: v = A'u / |A'u| with u = b/beta; w = v; x = 0.
-__global__ __launch_bounds__(1024) void vinit_kernel(int K, const double* rowp, const double* colp, State* S,
-                                                     double* v, double* w, double* x) {
+// Entries of the 2K-vector per workgroup of the element-parallel steps: 64
+// entries x 4 partial groups (256 threads); wave 0 finishes the entries.
+constexpr int VE = 64;
+
+// INIT (first direction, u = b/beta, beta from beta_kernel): v <- A'u / beta.
+// Else, with beta = |u| from the tiles' sums of squares (fixed_sum, the
+// same bits in every workgroup) and the current v (stored v times vscale):
+// the last scalar step's pending update of x and w, then v <- A'u / beta -
+// beta v (v unchanged when beta = 0).  Both with per-workgroup partial sums
+// of the squares of the new v (fixed order).
+template <bool INIT>
+__global__ __launch_bounds__(256) void stepa_kernel(int K, int ntiles, const double* normp, const double* rowp,
+                                                    const double* colp, const State* S, double* v, double* w,
+                                                    double* x, double* part) {
+    __shared__ double sh[4][VE];
+    __shared__ double sb[256];
     const int n2 = 2 * K;
+    if (S->done) return;
+    const double beta = INIT ? S->beta : sqrt(fixed_sum<256>(normp, ntiles, sb));
+    const int j = threadIdx.x & (VE - 1), g = threadIdx.x / VE;
+    const int i = blockIdx.x * VE + j;
+    sh[g][j] = i < n2 && beta > 0.0 ? atu_part(K, rowp, colp, i, g) : 0.0;
+    __syncthreads();
+    if (g == 0) {
+        double t = 0.0;
+        if (i < n2) {
+            const double a = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+            if (INIT) {
+                t = a * (1.0 / beta);
+            } else {
+                const double vn = v[i] * S->vscale;
+                if (S->upd) {
+                    const double wi = w[i];
+                    x[i] += S->t1 * wi;
+                    w[i] = vn + S->t2 * wi;
+                }
+                t = beta > 0.0 ? a * (1.0 / beta) - beta * vn : vn;
+            }
+            v[i] = t;
+        }
+        const double sq = wave_sum(t * t);
+        if (j == 0) part[blockIdx.x] = sq;
+    }
+}
+
+// First direction, scalar part (one workgroup): alpha = |A'u / beta| and the
+// state of iteration 0.
+__global__ __launch_bounds__(1024) void vinit_b_kernel(int nb, const double* part, State* S) {
     __shared__ double sh[1024];
-    if (S->done) {
-        for (int i = threadIdx.x; i < n2; i += 1024) x[i] = 0.0;
-        return;
-    }
-    const double ib = 1.0 / S->beta;
+    if (S->done) return;
     double s = 0.0;
-    for (int i = threadIdx.x; i < n2; i += 1024) {
-        const double t = atu_value(K, rowp, colp, i) * ib;
-        v[i] = t;
-        s += t * t;
-    }
+    for (int i = threadIdx.x; i < nb; i += 1024) s += part[i];
     s = block_sum<1024>(s, sh);
-    const double alpha = sqrt(s);
-    const double ia = alpha > 0.0 ? 1.0 / alpha : 0.0;
-    for (int i = threadIdx.x; i < n2; i += 1024) {
-        const double t = v[i] * ia;
-        v[i] = t;
-        w[i] = t;
-        x[i] = 0.0;
-    }
     if (threadIdx.x == 0) {
+        const double alpha = sqrt(s);
         S->alpha = alpha;
         S->phibar = S->beta;
         S->rhobar = alpha;
@@ -260,6 +330,9 @@ __global__ __launch_bounds__(1024) void vinit_kernel(int K, const double* rowp, 
         S->rnorm = S->beta;
         S->arnorm = alpha * S->beta;
         S->itn = 0;
+        S->vset = 1;
+        S->vscale = 1.0;
+        S->upd = 0;
         if (alpha == 0.0) {  // A'b = 0: x = 0 is a least-squares solution
             S->done = 1;
             S->flag = 0;
@@ -267,38 +340,35 @@ __global__ __launch_bounds__(1024) void vinit_kernel(int K, const double* rowp, 
     }
 }
 
-// One LSQR step after the new u and beta, in three launches:
-//  stepa (one thread per element): v <- A'u / beta - beta v, and per-block
-//    partial sums of its squares (fixed order);
-//  stepb (one workgroup): alpha = |v|, the damping and plane rotations, the
-//    stopping tests; the update's coefficients go to the state;
-//  stepc (one thread per element): v <- v / alpha, x and w.
-// beta = 0 (b in the range of A): v and alpha stay, the rotation below makes
-// the last update of x and phibar becomes 0.  stepc applies the update of
-// the iteration whose tests pass, as the last one.
-__global__ __launch_bounds__(256) void stepa_kernel(int K, const double* rowp, const double* colp, const State* S,
-                                                    double* v, double* part) {
-    const int n2 = 2 * K;
-    __shared__ double sh[256];
-    if (S->done || !(S->beta > 0.0)) return;
-    const double beta = S->beta, ib = 1.0 / beta;
+// First direction, element part: x = 0; v <- v / alpha (0 when alpha = 0)
+// and w = v when v was formed.
+__global__ __launch_bounds__(256) void vinit_c_kernel(int n2, const State* S, double* v, double* w, double* x) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    double t = 0.0;
-    if (i < n2) {
-        t = atu_value(K, rowp, colp, i) * ib - beta * v[i];
+    if (i >= n2) return;
+    x[i] = 0.0;
+    if (S->vset) {
+        const double ia = S->alpha > 0.0 ? 1.0 / S->alpha : 0.0;
+        const double t = v[i] * ia;
         v[i] = t;
+        w[i] = t;
     }
-    const double sq = block_sum<256>(t * t, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = sq;
 }
 
-__global__ __launch_bounds__(1024) void stepb_kernel(int nb, const double* part, State* S) {
+// One LSQR iteration after the pass, in two launches:
+//  stepa (above; 4 threads per element): the previous step's update of x
+//    and w, v <- A'u / beta - beta v, per-workgroup partial norms;
+//  stepb (one workgroup): beta = |u| (the same sum as stepa's) and the
+//    running Frobenius estimate of A, alpha = |v|, the damping and plane
+//    rotations, the stopping tests; the update's coefficients and the
+//    normalization 1/alpha of v go to the state, applied by the next
+//    pass and stepa (or finish_kernel after the last iteration).
+// beta = 0 (b in the range of A): v and alpha stay, the rotation below makes
+// the last update of x and phibar becomes 0.
+__global__ __launch_bounds__(1024) void stepb_kernel(int ntiles, const double* normp, int nb, const double* part,
+                                                     State* S) {
     __shared__ double sh[1024];
-    if (S->done) {
-        if (threadIdx.x == 0) S->upd = 0;
-        return;
-    }
-    const double beta = S->beta;
+    if (S->done) return;
+    const double beta = sqrt(fixed_sum<1024>(normp, ntiles, sh));
     double alpha = S->alpha;
     if (beta > 0.0) {
         double sq = 0.0;
@@ -307,6 +377,8 @@ __global__ __launch_bounds__(1024) void stepb_kernel(int nb, const double* part,
         alpha = sqrt(sq);
     }
     if (threadIdx.x == 0) {
+        S->beta = beta;
+        if (beta > 0.0) S->anorm = sqrt(S->anorm * S->anorm + S->alpha * S->alpha + beta * beta + S->damp * S->damp);
         const double dampsq = S->damp * S->damp;
         const double rhobar1 = sqrt(S->rhobar * S->rhobar + dampsq);
         const double cs1 = S->rhobar / rhobar1, sn1 = S->damp / rhobar1;
@@ -321,7 +393,7 @@ __global__ __launch_bounds__(1024) void stepb_kernel(int nb, const double* part,
         const double tau = sn * phi;
         S->t1 = phi / rho;
         S->t2 = -theta / rho;
-        S->ia = beta > 0.0 ? (alpha > 0.0 ? 1.0 / alpha : 0.0) : 1.0;
+        S->vscale = beta > 0.0 ? (alpha > 0.0 ? 1.0 / alpha : 0.0) : 1.0;
         S->upd = 1;
         S->alpha = alpha;
         S->res2 += psi * psi;
@@ -338,17 +410,11 @@ __global__ __launch_bounds__(1024) void stepb_kernel(int nb, const double* part,
     }
 }
 
-__global__ __launch_bounds__(256) void stepc_kernel(int n2, const State* S, double* v, double* w, double* x) {
+// After the last iteration: its pending update of x (x += t1 w).
+__global__ __launch_bounds__(256) void finish_kernel(int n2, const State* S, const double* w, double* x) {
     if (!S->upd) return;
-    const double t1 = S->t1, t2 = S->t2, ia = S->ia;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n2) {
-        const double vi = v[i] * ia;
-        v[i] = vi;
-        const double wi = w[i];
-        x[i] += t1 * wi;
-        w[i] = vi + t2 * wi;
-    }
+    if (i < n2) x[i] += S->t1 * w[i];
 }
 
 }  // namespace lsq
@@ -384,9 +450,9 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
         c->ls_ntiles = (int)h.size();
     }
     const int ntiles = c->ls_ntiles;
-    const int nstep = (n2 + 255) / 256;
+    const int nstep = (n2 + VE - 1) / VE, nelem = (n2 + 255) / 256;
     if ((rc = ensure(c, c->ls_pack, (size_t)P * 4)) || (rc = ensure(c, c->ls_u, (size_t)P * 8)) ||
-        (rc = ensure(c, c->ls_part, (size_t)std::max(ntiles, nstep) * 8)) ||
+        (rc = ensure(c, c->ls_part, (size_t)(ntiles + nstep) * 8)) ||
         (rc = ensure(c, c->ls_vec, (size_t)n2 * (2 + ncb + nkb) * 8)) ||
         (rc = ensure(c, c->ls_tab, (size_t)(2 * L + 2) * 5 * 8)) || (rc = ensure(c, c->ls_state, sizeof(State))))
         return rc;
@@ -397,7 +463,8 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     State* S = (State*)c->ls_state.p;
     unsigned* pack = (unsigned*)c->ls_pack.p;
     double* u = (double*)c->ls_u.p;
-    double* part = (double*)c->ls_part.p;
+    double* part = (double*)c->ls_part.p;  // the tiles' sums of squares of u
+    double* part2 = part + ntiles;         // stepa's per-workgroup sums of squares of v
     const int2* tiles = (const int2*)c->ls_tiles.p;
     State h{};
     h.damp = damp;
@@ -410,18 +477,18 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     tables_kernel<<<(2 * L + 2 + 255) / 256, 256, 0, c->stream>>>(L, ta, tb);
     pass_kernel<true><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, d_cl1, d_cl2, d_shift1d, pack, ta, tb, v, S, u, part,
                                                      rowp, colp);
-    beta_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, S, 1);
-    vinit_kernel<<<1, 1024, 0, c->stream>>>(K, rowp, colp, S, v, w, d_x);
+    beta_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, S);
+    stepa_kernel<true><<<nstep, 256, 0, c->stream>>>(K, ntiles, part, rowp, colp, S, v, w, d_x, part2);
+    vinit_b_kernel<<<1, 1024, 0, c->stream>>>(nstep, part2, S);
+    vinit_c_kernel<<<nelem, 256, 0, c->stream>>>(n2, S, v, w, d_x);
     HIPCHK(c, hipGetLastError());
     // launches after the stopping tests pass return at once; the done flag is
     // read back every 8 iterations so a converged solve stops issuing them
     for (int it = 0; it < maxit; it++) {
         pass_kernel<false><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, nullptr, nullptr, nullptr, pack, ta, tb, v, S, u,
                                                           part, rowp, colp);
-        beta_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, S, 0);
-        stepa_kernel<<<nstep, 256, 0, c->stream>>>(K, rowp, colp, S, v, part);
-        stepb_kernel<<<1, 1024, 0, c->stream>>>(nstep, part, S);
-        stepc_kernel<<<nstep, 256, 0, c->stream>>>(n2, S, v, w, d_x);
+        stepa_kernel<false><<<nstep, 256, 0, c->stream>>>(K, ntiles, part, rowp, colp, S, v, w, d_x, part2);
+        stepb_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, nstep, part2, S);
         if (it % 8 == 7 && it + 1 < maxit) {
             int done = 0;
             HIPCHK(c, hipGetLastError());
@@ -430,6 +497,7 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
             if (done) break;
         }
     }
+    finish_kernel<<<nelem, 256, 0, c->stream>>>(n2, S, w, d_x);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&h, S, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));

@@ -152,8 +152,11 @@ struct TriArgs {
     const double* thr;          // [n_intervals-1] thresholds (repeated additions of 1/n)
     double* cum;                // [P] or null
     unsigned long long* hist;   // [n_intervals] counts
-    // PROBS: P2, A, a, B, b, x0
+    // PROBS: P2, A, a, B, b, x0; and, when lform, log A, log(P2 B),
+    // log((1 - P2) A), -b/(1 - x0)
     double prm[6];
+    double lg[4];
+    int lform;
     double* ind;
     double* arb;
     // SIGNS
@@ -182,6 +185,27 @@ __device__ inline double f_ind(double x, const double* prm) {
 }
 
 __device__ inline double f_arb(double x, const double* prm) { return log(prm[1] * pow(1 - x, prm[2])); }
+
+// Both log-likelihoods of one score from one logarithm, lu = log(1 - x), for
+// A, B > 0, a, b > 0 and 0 <= P2 <= 1 (the same functions; the direct form
+// above spends three powers, an exponential and two logarithms):
+//   arb = log A + a lu
+//   ind = log(e^t1 + e^t2),  t1 = log(P2 B) + b lu - b/(1-x0) (1-x),
+//                            t2 = log((1-P2) A) + a lu
+// with log(e^t1 + e^t2) = hi + log1p(exp(lo - hi)).
+__device__ inline void f_both(double x, const double* prm, const double* lg, double& ind, double& arb) {
+    const double u = 1 - x;
+    const double lu = log(u);
+    const double t1 = lg[1] + prm[4] * lu + lg[3] * u;
+    const double t2 = lg[2] + prm[2] * lu;
+    arb = lg[0] + prm[2] * lu;
+    if (t1 != t1 || t2 != t2) {
+        ind = t1 + t2;  // NaN
+        return;
+    }
+    const double hi = t1 > t2 ? t1 : t2, lo = t1 > t2 ? t2 : t1;
+    ind = hi == -INFINITY ? hi : hi + log1p(exp(lo - hi));
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256) void tri_kernel(TriArgs a) {
@@ -265,8 +289,16 @@ __global__ __launch_bounds__(256) void tri_kernel(TriArgs a) {
                     const double hs = role == 0 ? sa : (role == 1 ? sb : sb);
                     atomicAdd(&s_hist[bin_of(hs, a.thr, a.n_intervals)], 1u);
                 } else if constexpr (MODE == PROBS) {
-                    acc0 += f_ind(sa, a.prm) + f_ind(sb, a.prm);
-                    acc1 += f_arb(sa, a.prm) + f_arb(sb, a.prm);
+                    if (a.lform) {
+                        double ia, aa, ib, ab;
+                        f_both(sa, a.prm, a.lg, ia, aa);
+                        f_both(sb, a.prm, a.lg, ib, ab);
+                        acc0 += ia + ib;
+                        acc1 += aa + ab;
+                    } else {
+                        acc0 += f_ind(sa, a.prm) + f_ind(sb, a.prm);
+                        acc1 += f_arb(sa, a.prm) + f_arb(sb, a.prm);
+                    }
                 } else {
                     double ea, eb;
                     const double sga = SGN[best][qa], sgb = SGN[best][qb];
@@ -440,6 +472,14 @@ extern "C" int clm_sync3n_pairs_probabilities(clm_ctx* c, int32_t N, const doubl
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     for (int i = 0; i < 6; i++) a.prm[i] = prm[i];
+    {
+        const double P2 = prm[0], A = prm[1], ea = prm[2], B = prm[3], eb = prm[4], x0 = prm[5];
+        a.lform = A > 0 && B > 0 && ea > 0 && eb > 0 && P2 >= 0 && P2 <= 1 && x0 != 1;
+        a.lg[0] = std::log(A);
+        a.lg[1] = std::log(P2 * B);
+        a.lg[2] = std::log((1 - P2) * A);
+        a.lg[3] = -eb / (1 - x0);
+    }
     a.ind = d_ln_f_ind;
     a.arb = d_ln_f_arb;
     if ((rc = s3_launch<PROBS>(c, a, 0))) return rc;

@@ -392,7 +392,7 @@ def evaluate(net: "ShiftNet", K=5000, seed=1005, snr=1 / 16, n=129, n_r=65, max_
 
     st = sim.make_stack(K=K, n=n, n_r=n_r, n_theta=T, snr=snr, max_shift_2d=max_shift, shift_kind="uniform",
                         seed=seed, device="cuda")
-    gate = float(net.meta.get("gate", 0.0)) if gate is None else float(gate)
+    gate = net_gate(net, snr) if gate is None else float(gate)
     P = K * (K - 1) // 2
     torch.cuda.synchronize()
     # warm both paths once (kernel code, buffers), then time
@@ -465,13 +465,29 @@ def _cmd_train(args):
     print(json.dumps(info))
 
 
+def snr_key(snr: float) -> str:
+    return "%g" % snr
+
+
+def net_gate(net: "ShiftNet", snr: float) -> float:
+    """The window gate stored with the weights for data of this SNR (the
+    calibration of that SNR, else the default ``gate``, else 0).  Window
+    peaks scale with the SNR, so a gate holds for the SNR it was calibrated
+    on."""
+    gates = net.meta.get("gates", {})
+    if snr_key(snr) in gates:
+        return float(gates[snr_key(snr)])
+    return float(net.meta.get("gate", 0.0))
+
+
 def _cmd_calibrate(net: "ShiftNet", args) -> dict:
     """The gate: on a held-out stack (seed outside the training seeds, images
     of the learned workload's model: 129 px, shifts U[-12,12], 49-shift full
     grid, SNR 1 by default), the smallest window peak at which the combined
     search finds at least (1 - max_detect_loss) of the true common lines the
     full grid finds (:func:`calibrate_gate` with the true rotations).
-    Stored in the weights' meta as ``gate``."""
+    Stored in the weights' meta under ``gates`` for that SNR; the SNR-1
+    calibration is also the default ``gate``."""
     from . import sim
     from .clmatrix import clmatrix_records
 
@@ -486,8 +502,14 @@ def _cmd_calibrate(net: "ShiftNet", args) -> dict:
     info = dict(gate=gate, K=args.K, snr=args.snr, seed=args.seed,
                 criterion="agreement >= %g" % args.target if args.no_truth else
                 "detection loss <= %g" % args.max_detect_loss, table=table, window_stats=ws)
-    net.meta["gate"] = float(gate)
-    net.meta["gate_calibration"] = {k: v for k, v in info.items() if k not in ("table", "window_stats")}
+    net.meta.setdefault("gates", {})[snr_key(args.snr)] = float(gate)
+    cal = net.meta.get("gate_calibration", {})
+    if "gate" in cal:  # one calibration stored flat: keep it under its SNR
+        cal = {snr_key(cal.get("snr", 1.0)): cal}
+    cal[snr_key(args.snr)] = {k: v for k, v in info.items() if k not in ("table", "window_stats")}
+    net.meta["gate_calibration"] = cal
+    if args.snr == 1.0:
+        net.meta["gate"] = float(gate)
     return info
 
 

@@ -533,6 +533,7 @@ def pft_bench(args):
     cpu_sec = time.perf_counter() - t
     got = pf[:, :, torch.as_tensor(sample, device=pf.device)].cpu().numpy()
     rel = float(np.abs(got - ora).max() / np.abs(ora).max())
+    nufft = nufft_bench(args, p, pf, n_r, T, sample, ora)
     out = {"metric": "images/s: cryo_pft n=129 n_r=65 n_theta=360 (exact NUDFT, FP64)", "value": K / sec,
            "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * sec, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
@@ -545,16 +546,65 @@ def pft_bench(args):
            "cpu_baseline": {"value": ns / cpu_sec, "unit": "images/s", "cores": nth, "kind": "port",
                             "sample": f"{ns} images of the workload, FP64 C oracle (OpenMP)"},
            "parity_sample": {"images": ns, "max_abs_err_rel_to_max": rel},
-           "torch_zgemm": {"seconds": t_torch, "images_per_s": K / t_torch, "max_diff_rel_to_max": zdiff}}
+           "torch_zgemm": {"seconds": t_torch, "images_per_s": K / t_torch, "max_diff_rel_to_max": zdiff},
+           "nufft": nufft}
     print(json.dumps(out))
+
+
+def nufft_bench(args, p, pf_exact, n_r, T, sample, ora):
+    """cryo_pft's NUFFT method (CLM_PFT_NUFFT) on the same images: time,
+    stage breakdown, difference from the exact method over all images and
+    from the oracle on its sample; then both methods on 400 images of side
+    359 (n_r=180), where the exact sums cost 21x more per image."""
+    import torch
+
+    from aspire_amd.pft import cryo_pft
+
+    K = p.shape[2]
+
+    def timed(q, nr, method, steps):
+        out, _ = cryo_pft(q, nr, T, method=method)  # warm-up (FFT plans)
+        ts, sts = [], []
+        for _ in range(steps):
+            del out
+            torch.cuda.synchronize()
+            st = {}
+            t = time.perf_counter()
+            out, _ = cryo_pft(q, nr, T, stats=st, method=method)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+            sts.append(st)
+        return out, float(np.mean(ts)), sts[-1]
+
+    pfn, sec, st = timed(p, n_r, "nufft", args.steps)
+    diff = (pfn - pf_exact).abs().max().item() / pf_exact.abs().max().item()
+    got = pfn[:, :, torch.as_tensor(sample, device=pfn.device)].cpu().numpy()
+    rel = float(np.abs(got - ora).max() / np.abs(ora).max())
+    del pfn
+    log(f"pft nufft K={K}: {sec:.4f}s {st} vs exact {diff:.2e}, vs oracle {rel:.2e}")
+    n2, nr2, K2 = 359, 180, 400
+    g = torch.Generator(device=p.device).manual_seed(359)
+    q = torch.randn((K2, n2, n2), dtype=torch.float64, device=p.device, generator=g).permute(2, 1, 0)
+    a, sec_e, st_e = timed(q, nr2, "exact", 1)
+    b, sec_n, st_n = timed(q, nr2, "nufft", 1)
+    diff2 = (a - b).abs().max().item() / a.abs().max().item()
+    del a, b, q
+    log(f"pft n={n2} K={K2}: exact {sec_e:.4f}s, nufft {sec_n:.4f}s, diff {diff2:.2e}")
+    return {"seconds": sec, "images_per_s": K / sec, "stats": st, "max_diff_vs_exact_rel_to_max": diff,
+            "max_abs_err_vs_oracle_rel_to_max": rel, "oracle_images": len(sample),
+            "large": {"n": n2, "n_r": nr2, "n_theta": T, "K": K2, "images": "standard normal pixels",
+                      "exact_seconds": sec_e, "nufft_seconds": sec_n, "speedup": sec_e / sec_n,
+                      "max_diff_rel_to_max": diff2, "exact_stats": st_e, "nufft_stats": st_n}}
 
 
 def shifts_bench(args):
     """2-D shifts by LSQR (tol 1e-8, maxit 100: cryo_estimate_shifts.m:229)
     on the shift equations of the K=5000 workload, computed on the device
     from the BF16 engine's per-pair outputs.  HBM-bound: the algorithmic
-    traffic is 32 bytes per pair per iteration (packed lines read by A*v
-    and A'*u, u read twice and written once).  CPU baseline: the oracle's
+    traffic is 20 bytes per pair per iteration (one pass forms u' = A v -
+    (alpha/beta) u and A'u' together: the packed lines and u read once, u
+    written once; a solver with separate A v and A'u passes moves 32).
+    CPU baseline: the oracle's
     LSQR (scipy CSR products) on the same full system, a few iterations
     timed and scaled to the iterations the device ran."""
     import torch
@@ -585,7 +635,7 @@ def shifts_bench(args):
     sec = float(np.mean(times))
     it = st["iterations"]
     dev_ms = float(np.mean([s_["ms"] for s_ in sts]))
-    achieved = 32.0 * P * it / (dev_ms * 1e-3)
+    achieved = 20.0 * P * it / (dev_ms * 1e-3)
     # measured HBM bytes of one solve (rocprof FETCH_SIZE/WRITE_SIZE, calibrated),
     # valid for the workload and iteration count it was measured on
     traffic = None
@@ -611,7 +661,7 @@ def shifts_bench(args):
            "lsqr": st,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
                         "frac": achieved / 8e12, "traffic": traffic, "kernel": "whole solve",
-                        "bytes_per_pair_iteration": 32},
+                        "bytes_per_pair_iteration": 20},
            "cpu_baseline": {"value": P / (cpu_it * it), "unit": "pairs/s", "cores": 1, "kind": "port",
                             "sample": f"{n_cpu} LSQR iterations of the oracle (scipy CSR) on the full system, "
                                       f"scaled to {it} iterations"},

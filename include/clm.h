@@ -364,6 +364,21 @@ enum clm_real_dtype {
     CLM_R32 = 1  /* single images (MATLAB single) */
 };
 
+/*
+ * Method: CLM_PFT_EXACT evaluates the sums above exactly (FP64 MFMA
+ * contraction, O(n^2) per polar point); CLM_PFT_NUFFT evaluates them by a
+ * type-2 non-uniform FFT, as nufft2 does with a NUFFT library installed
+ * (fourier/nufft/nufft_transform.m:108-150; epsilon 1e-15 by default,
+ * nufft_initialize.m:29): exponential-of-semicircle kernel of width 16 on
+ * a 2x-oversampled grid, hipFFT for the grid transforms, O(n^2 log n) per
+ * image plus 256 grid reads per polar point.  The two agree to about
+ * 1e-14 of max |pf|.  n <= 2048 for the NUFFT method.
+ */
+enum clm_pft_method {
+    CLM_PFT_EXACT = 0,
+    CLM_PFT_NUFFT = 1
+};
+
 typedef struct clm_pft_params {
     int32_t n;         /* image side (square images, nudft2.m:27-31)       */
     int32_t K;         /* number of images                                 */
@@ -371,13 +386,21 @@ typedef struct clm_pft_params {
     int32_t n_theta;   /* rays; must be even (cryo_pft.m:44-46)            */
     int32_t img_dtype; /* enum clm_real_dtype                              */
     int32_t pf_dtype;  /* enum clm_dtype of the output                     */
+    int32_t method;    /* enum clm_pft_method                              */
 } clm_pft_params;
 
 typedef struct clm_pft_stats {
-    float ms_tables; /* per-axis exponential tables                 */
-    float ms_stage;  /* images to the pixel-major FP64 staging area */
-    float ms_pft;    /* the contraction (FP64 MFMA)                 */
-    int32_t chunks;  /* image chunks of at most 8192                */
+    float ms_tables;   /* exact: per-axis exponential tables; NUFFT:
+                          kernel weights of the polar points           */
+    float ms_stage;    /* exact: images to the pixel-major FP64 staging
+                          area; NUFFT: deconvolution onto the grids     */
+    float ms_pft;      /* exact: the contraction (FP64 MFMA); NUFFT: the
+                          interpolation at the polar points            */
+    int32_t chunks;    /* image chunks (exact: at most 8192 images;
+                          NUFFT: at most 2048 images and 4 GiB of grid) */
+    float ms_fft;      /* NUFFT: the batched 2-D FFTs                   */
+    int32_t grid_side; /* NUFFT: oversampled grid side N (2,3,5-smooth,
+                          >= 2n)                                        */
 } clm_pft_stats;
 
 /* Device-buffer form: d_img and d_pf are device pointers on the context's

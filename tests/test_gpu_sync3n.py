@@ -67,11 +67,16 @@ def test_consistent_and_degenerate(oracle):
     assert np.array_equal(h, ho) and np.array_equal(c, co, equal_nan=True)
 
 
-@pytest.mark.parametrize("N", [3, 40])
-def test_pairs_probabilities(oracle, N):
-    say("case sync3n probabilities", N)
+# the device evaluates both log-likelihoods from one logarithm when A, B, a,
+# b > 0 and 0 <= P2 <= 1 (the first two rows), else the reference's direct
+# formula (a = 0; P2 = 1 leaves only the arbitrary model's term)
+PRMS = [(0.45, 1.7, 2.2, 2.9, 2.5, 0.78), (1.0, 1.7, 2.2, 2.9, 2.5, 0.78), (0.45, 1.7, 0.0, 2.9, 2.5, 0.78)]
+
+
+@pytest.mark.parametrize("N,prm", [(3, PRMS[0]), (40, PRMS[0]), (40, PRMS[1]), (40, PRMS[2])])
+def test_pairs_probabilities(oracle, N, prm):
+    say("case sync3n probabilities", N, prm)
     R, _ = noisy(N, 200 + N)
-    prm = (0.45, 1.7, 2.2, 2.9, 2.5, 0.78)
     ind, arb, _ = pairs_probabilities_mex(R, *prm)
     io, ao = oracle.pairs_probabilities(R, *prm)
     assert np.allclose(ind, io, rtol=1e-12, atol=0) and np.allclose(arb, ao, rtol=1e-12, atol=0)
