@@ -1333,7 +1333,9 @@ void clm_default_params(clm_params* p) {
     p->pf_dtype = CLM_C128;
     p->max_shift = 15.0;   // cryo_clmatrix_cpu.m:133
     p->shift_step = 1.0;   // :138
-    p->precision = CLM_BF16_RESCORE;
+    // FP16: the same MFMA rate as BF16 with an 8x tighter rounding bound,
+    // so broad high-SNR peaks stay certain without the FP32 refinement
+    p->precision = CLM_FP16_RESCORE;
     p->verbose = 1;        // :129
     p->tie_margin = 1e-3;
 }

@@ -9,7 +9,7 @@
  * form of cryo_clmatrix_cpu.m:764-772) from libclm's C ABI (include/clm.h):
  * clm_clmatrix (or clm_clmatrix_filtered for map_filter_radius > 0) on the
  * GPU, then clm_assemble and clm_shift_equations_csc on the host in FP64.
- * precision: 0 FP32, 1 BF16 + re-score (default), 2 FP16 + re-score, 3 FP64.
+ * precision: 0 FP32, 1 BF16 + re-score, 2 FP16 + re-score (default), 3 FP64.
  * clstack_mask is filled by clm_clstack_mask when verbose has bit 2 and a
  * K x K ref_clmatrix is given (cryo_clmatrix_cpu.m:567-617,697-698), else 0.
  * verbose bit 1 prints libclm's reports with mexPrintf; lines of near-zero
@@ -69,7 +69,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     p.verbose = (int32_t)scalar_arg(nrhs, prhs, 1, 1.0);
     p.max_shift = scalar_arg(nrhs, prhs, 2, 15.0);
     p.shift_step = scalar_arg(nrhs, prhs, 3, 1.0);
-    p.precision = (int32_t)scalar_arg(nrhs, prhs, 4, (double)CLM_BF16_RESCORE);
+    p.precision = (int32_t)scalar_arg(nrhs, prhs, 4, (double)CLM_FP16_RESCORE);
     const int32_t radius = (int32_t)scalar_arg(nrhs, prhs, 5, 0.0);
     const mxArray *ref = nrhs > 6 && !mxIsEmpty(prhs[6]) ? prhs[6] : NULL;
     if (ref && (!mxIsDouble(ref) || mxIsComplex(ref) || mxGetM(ref) != (size_t)p.K || mxGetN(ref) != (size_t)p.K))

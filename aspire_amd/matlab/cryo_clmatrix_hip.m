@@ -13,7 +13,7 @@ function [clstack,corrstack,shift_equations,shift_equations_map,clstack_mask]=..
 %                     cryo_average_clmap before the search (FP64).
 %   ref_shifts_2d     only feeds the reference's per-pair debug printing; it is
 %                     accepted and not used.
-% The search runs in BF16 on the matrix cores, re-scored in FP32 and, for
+% The search runs in FP16 on the matrix cores, re-scored in FP32 and, for
 % near ties, in FP64 (see clm_mex.c for the other precisions).
 
 if ~exist('verbose','var') || isempty(verbose), verbose=1; end
@@ -21,7 +21,7 @@ if ~exist('max_shift','var') || isempty(max_shift), max_shift=15; end
 if ~exist('shift_step','var') || isempty(shift_step), shift_step=1; end
 if ~exist('map_filter_radius','var') || isempty(map_filter_radius), map_filter_radius=0; end
 if ~exist('ref_clmatrix','var') || isscalar(ref_clmatrix), ref_clmatrix=[]; end
-precision=1;   % CLM_BF16_RESCORE
+precision=2;   % CLM_FP16_RESCORE
 
 [clstack,corrstack,shift_equations,shift_equations_map,clstack_mask]=...
     clm_mex(pf,verbose,max_shift,shift_step,precision,map_filter_radius,ref_clmatrix);

@@ -266,7 +266,7 @@ def train(net: ShiftNet, x, y, epochs=40, batch=256, lr=1e-3, val_frac=0.1, seed
     return hist
 
 
-def clmatrix_windowed(pf, prop, max_shift=12.0, shift_step=0.5, window=1.0, gate=0.0, precision="bf16",
+def clmatrix_windowed(pf, prop, max_shift=12.0, shift_step=0.5, window=1.0, gate=0.0, precision="fp16",
                       device=0, variant=0, flags=0):
     """Windowed search with exhaustive fallback (libclm's
     clm_clmatrix_windowed).  ``prop``: K x 2 float32 device tensor of
@@ -300,7 +300,7 @@ def clmatrix_windowed(pf, prop, max_shift=12.0, shift_step=0.5, window=1.0, gate
 
 
 def calibrate_gate(pf, prop, full_rec, max_shift=12.0, shift_step=0.5, window=1.0, target=0.999,
-                   precision="bf16", rots=None, max_detect_loss=0.005):
+                   precision="fp16", rots=None, max_detect_loss=0.005):
     """The window gate from a stack whose full-grid result ``full_rec``
     (host arrays) is known.  Returns (gate, table of candidates).
 
@@ -380,7 +380,7 @@ def pair_shift_error(prop, shifts, rots, L=T // 2, n_pairs=20000, seed=0):
 
 
 def evaluate(net: "ShiftNet", K=5000, seed=1005, snr=1 / 16, n=129, n_r=65, max_shift=12.0, shift_step=0.5,
-             window=1.0, gate=None, log=print):
+             window=1.0, gate=None, log=print, precision="fp16"):
     """The learned form on one stack: full-grid search against the windowed
     search with the network's proposals and with zero proposals (the
     method without the network): time, agreement with the full grid,
@@ -396,27 +396,27 @@ def evaluate(net: "ShiftNet", K=5000, seed=1005, snr=1 / 16, n=129, n_r=65, max_
     P = K * (K - 1) // 2
     torch.cuda.synchronize()
     # warm both paths once (kernel code, buffers), then time
-    clmatrix_records(st.pf, max_shift, shift_step, precision="bf16")
+    clmatrix_records(st.pf, max_shift, shift_step, precision=precision)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    full, fst = clmatrix_records(st.pf, max_shift, shift_step, precision="bf16")
+    full, fst = clmatrix_records(st.pf, max_shift, shift_step, precision=precision)
     torch.cuda.synchronize()
     t_full = time.perf_counter() - t
     full = {k: v.cpu().numpy() for k, v in full.items()}
     prop = net.propose(st.pf)
-    clmatrix_windowed(st.pf, prop, max_shift, shift_step, window, gate)
+    clmatrix_windowed(st.pf, prop, max_shift, shift_step, window, gate, precision=precision)
     torch.cuda.synchronize()
     t = time.perf_counter()
     prop = net.propose(st.pf)
     torch.cuda.synchronize()
     t_net = time.perf_counter() - t
     t = time.perf_counter()
-    rec, ws = clmatrix_windowed(st.pf, prop, max_shift, shift_step, window, gate)
+    rec, ws = clmatrix_windowed(st.pf, prop, max_shift, shift_step, window, gate, precision=precision)
     torch.cuda.synchronize()
     t_win = time.perf_counter() - t
     rec = {k: v.cpu().numpy() for k, v in rec.items()}
     zero = torch.zeros_like(prop)
-    rz, wz = clmatrix_windowed(st.pf, zero, max_shift, shift_step, window, gate)
+    rz, wz = clmatrix_windowed(st.pf, zero, max_shift, shift_step, window, gate, precision=precision)
     rz = {k: v.cpu().numpy() for k, v in rz.items()}
 
     def agree(r):
@@ -437,7 +437,7 @@ def evaluate(net: "ShiftNet", K=5000, seed=1005, snr=1 / 16, n=129, n_r=65, max_
     out = dict(K=K, pairs=P, snr=snr, seed=seed, full_shifts=ws["full_shifts"], window_shifts=ws["window_shifts"],
                gate=gate, full_seconds=t_full, full_pairs_per_s=P / t_full,
                net_seconds=t_net, windowed_seconds=t_win, learned_pairs_per_s=P / (t_net + t_win),
-               speedup=t_full / (t_net + t_win),
+               speedup=t_full / (t_net + t_win), precision=precision,
                settled=ws["accepted"] / P, below_gate=ws["below_gate"] / P, off_grid=ws["off_grid"] / P,
                agree_full=agree(rec), agree_full_zero_proposals=agree(rz),
                settled_zero_proposals=wz["accepted"] / P,
@@ -493,7 +493,7 @@ def _cmd_calibrate(net: "ShiftNet", args) -> dict:
 
     st = sim.make_stack(K=args.K, n=129, n_r=65, n_theta=T, snr=args.snr, max_shift_2d=12.0, shift_kind="uniform",
                         seed=args.seed, device="cuda")
-    full, _ = clmatrix_records(st.pf, 12.0, 0.5, precision="bf16")
+    full, _ = clmatrix_records(st.pf, 12.0, 0.5, precision="fp16")
     full = {k: v.cpu().numpy() for k, v in full.items()}
     prop = net.propose(st.pf)
     gate, table = calibrate_gate(st.pf, prop, full, 12.0, 0.5, 1.0, target=args.target,

@@ -152,6 +152,22 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    # FP16 + re-score (the drop-in's default precision) on the same stack:
+    # the same MFMA rate, an 8x tighter rounding bound
+    fp16 = None
+    rec16 = None
+    if args.precision != "fp16":
+        run("fp16", pf)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rec16, st16 = run("fp16", pf)
+        torch.cuda.synchronize()
+        sec16 = time.perf_counter() - t
+        fp16 = {"value": P / sec16, "unit": "pairs/s", "seconds": sec16, "ms_pairs": st16["ms_pairs"],
+                "ms_rescore": st16["ms_rescore"], "variant": st16["variant"], "cells_rescored": st16["cells_rescored"],
+                "uncertain_pairs": st16["uncertain_pairs"]}
+        log(f"fp16: {sec16:.3f}s {st16}")
+
     # plain FP32 GPU path on the same stack (the first fp32-K images; all by
     # default).  The autotuner's timing launches warm its kernels; the one
     # timed call includes its own buffer growth, a few ms against ~50 s.
@@ -201,6 +217,15 @@ def main():
     corr_rel = float(np.max(np.abs(g["corr"][m] - ora["best"][m]) / np.abs(ora["best"][m]))) if m.any() else 0.0
     log(f"parity sample {ns}: agree {same.mean():.5f}, near-tie disagreements {near}/{(~same).sum()}, "
         f"corr rel {corr_rel:.2e}, oracle {cpu_sec:.1f}s on {nth} threads")
+    if rec16 is not None:
+        g16 = {k: v.cpu().numpy()[slots] for k, v in rec16.items()}
+        s16 = (g16["cl1"] == ora["cl1"]) & (g16["cl2"] == ora["cl2"]) & (g16["j"] == ora["j"])
+        m16 = s16 & (ora["best"] != 0)
+        fp16["parity_sample"] = {
+            "pairs": int(ns), "disagree": int((~s16).sum()),
+            "corr_rel_max": float(np.max(np.abs(g16["corr"][m16] - ora["best"][m16]) / np.abs(ora["best"][m16])))
+            if m16.any() else 0.0}
+        del rec16
 
     # learned form on its own workload: K=5000, shifts U[-12,12], full grid
     # max_shift 12 step 0.5 (49 shifts) against the windowed search (5 shifts)
@@ -259,6 +284,7 @@ def main():
                      "kernel": "pair_kernel (pass 1)", "kernel_ms": k_ms,
                      "flops_per_pair": W_pair, "executed_flops_per_pair": W_pair / 2},
         "cpu_baseline": cpu,
+        "fp16": fp16,
         "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "seconds": fp32_sec,
                      "ms_pairs": st32["ms_pairs"], "variant": st32["variant"], "autotune": tune32},
         "dropin": dropin,

@@ -34,7 +34,7 @@ def test_version_defaults_and_sizes():
     assert L.clm_version() == 10000
     p = _lib.default_params()
     assert p.max_shift == 15.0 and p.shift_step == 1.0 and p.verbose == 1  # cryo_clmatrix_cpu.m:128-139
-    assert p.tie_margin == 1e-3 and p.precision == _lib.CLM_BF16_RESCORE
+    assert p.tie_margin == 1e-3 and p.precision == _lib.CLM_FP16_RESCORE
     assert ctypes.sizeof(_lib.ClmParams) == 56
     assert L.clm_num_pairs(5000) == 12497500 and L.clm_num_pairs(1) == 0
 

@@ -136,7 +136,7 @@ def test_windowed_search_properties(oracle, shifted):
     rng = np.random.default_rng(0)
     prop_np = (shifted.shifts + rng.normal(0, 0.3, size=shifted.shifts.shape)).astype(np.float32)
     prop = torch.as_tensor(prop_np).cuda()
-    rec, ws = clmatrix_windowed(to_dev(pf), prop, 12, 0.5, window=1.0, gate=0.0)
+    rec, ws = clmatrix_windowed(to_dev(pf), prop, 12, 0.5, window=1.0, gate=0.0, precision="bf16")
     say(ws)
     w = {k: v.cpu().numpy() for k, v in rec.items()}
     assert ws["window_shifts"] == 5 and ws["full_shifts"] == 49

@@ -48,7 +48,7 @@ def test_trained_network_gain(snr):
     full = {k: v.cpu().numpy() for k, v in full.items()}
     res = {}
     for name, p in (("net", prop), ("zero", torch.zeros_like(prop))):
-        rec, ws = clmatrix_windowed(st.pf, p, 12, 0.5, window=1.0, gate=0.0)
+        rec, ws = clmatrix_windowed(st.pf, p, 12, 0.5, window=1.0, gate=0.0, precision="bf16")
         r = {k: v.cpu().numpy() for k, v in rec.items()}
         agree = float(np.mean((r["cl1"] == full["cl1"]) & (r["cl2"] == full["cl2"]) & (r["j"] == full["j"])))
         res[name] = (agree, r)

@@ -135,10 +135,10 @@ def test_learned_workload(oracle):
     net = ShiftNet.load()
     prop = net.propose(st.pf)
     # a gate at the median window peak of the settled pairs hands back about half of them
-    rec0, ws0 = clmatrix_windowed(st.pf, prop, w.max_shift, w.shift_step, 1.0, gate=-1e30)
+    rec0, ws0 = clmatrix_windowed(st.pf, prop, w.max_shift, w.shift_step, 1.0, gate=-1e30, precision="bf16")
     ex0 = rec0["exhaustive"].cpu().numpy() == 1
     gate = float(np.median(rec0["corr"].cpu().numpy()[~ex0]))
-    rec, ws = clmatrix_windowed(st.pf, prop, w.max_shift, w.shift_step, 1.0, gate=gate)
+    rec, ws = clmatrix_windowed(st.pf, prop, w.max_shift, w.shift_step, 1.0, gate=gate, precision="bf16")
     say("gate", gate, ws)
     assert ws["window_shifts"] == 5 and ws["full_shifts"] == 49
     assert ws["accepted"] + ws["below_gate"] + ws["off_grid"] == w.pairs
