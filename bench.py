@@ -767,7 +767,8 @@ def sync3n_bench(args):
     FP64 vector; the reference's arithmetic is 297 flops per triangle (four
     3x3 products, four squared norms, three divisions, square roots and
     subtractions), which the per-pair kernel evaluates three times (once per
-    side, 249 flops each: three products).  CPU baseline: the oracle's split
+    side; rocprof counts 659 executed FP64 operations per triangle, with the
+    division and square-root sequences).  CPU baseline: the oracle's split
     of the triangle loop over threads (the reference's) on the first 600
     images; parity: the GPU against the one-thread oracle on those images."""
     import torch
@@ -834,6 +835,11 @@ def sync3n_bench(args):
     clm_oracle.triangles_scores(Rsub, 100, nthreads=nth)
     cpu_s = time.perf_counter() - t
     Ts = Ns * (Ns - 1) * (Ns - 2) // 6
+    # measured HBM bytes of the scores kernel at N=5000 (rocprof, calibrated)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sync3n_k5000_traffic.json")
+    if os.path.exists(tpath) and N == 5000:
+        traffic = json.load(open(tpath))["hbm_bytes_per_launch"]
     out = {"metric": "triangles/s: sync3n triangles_scores_mex, N=5000 relative rotations", "value": T / sec,
            "unit": "triangles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * sec, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
@@ -844,9 +850,9 @@ def sync3n_bench(args):
            "pairs_probabilities": {"seconds": sec_p, "triangles_per_s": T / sec_p, "stats": stp},
            "signs_times_v": {"seconds": sec_v, "triangles_per_s": T / sec_v, "n_eigs": 4, "stats": stv},
            "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": 78.6, "unit": "TFLOP/s",
-                        "frac": achieved / 78.6e12, "traffic": None, "kernel": "tri_kernel<SCORES>",
+                        "frac": achieved / 78.6e12, "traffic": traffic, "kernel": "tri_kernel<SCORES>",
                         "kernel_ms": st["ms_kernel"], "flops_per_triangle": 297,
-                        "executed_flops_per_triangle": 3 * 249},
+                        "executed_flops_per_triangle": 659},
            "cpu_baseline": {"value": Ts / cpu_s, "unit": "triangles/s", "cores": int(nth), "kind": "port",
                             "sample": f"N={Ns} (first images), the oracle's threaded split (the reference's)"},
            "parity_sample": {"N": Ns, "hist_equal": bool(np.array_equal(hg, ho)),
