@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace clm;
@@ -1323,6 +1324,33 @@ struct NoAllocScope {
 };
 }  // namespace
 
+namespace {
+// f(begin, end) over [0, n) split into contiguous ranges, one per host
+// thread (at most 32, at least `grain` items each); runs inline when a
+// thread cannot be started.  Nothing here throws across the C ABI.
+template <typename F>
+void host_parallel(long long n, long long grain, F f) {
+    if (n <= 0) return;
+    const long long hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::max<long long>(1, std::min<long long>({hw, 32LL, n / std::max(grain, 1LL)}));
+    std::vector<std::thread> th;
+    int started = 0;
+    if (nt > 1) {
+        try {
+            for (int t = 1; t < nt; t++) {
+                th.emplace_back([&f, n, nt, t] { f(n * t / nt, n * (t + 1) / nt); });
+                started = t;
+            }
+        } catch (...) {
+        }
+    }
+    // this thread: range 0, then every range whose thread did not start
+    f(0, n / nt);
+    for (int t = started + 1; t < nt; t++) f(n * t / nt, n * (t + 1) / nt);
+    for (auto& x : th) x.join();
+}
+}  // namespace
+
 extern "C" {
 
 int clm_version(void) { return CLM_VERSION; }
@@ -1577,21 +1605,28 @@ int clm_clstack_mask(int32_t K, int32_t n_theta, const double* clstack, const do
 int clm_assemble(int32_t K, const int32_t* cl1, const int32_t* cl2, const double* corr, double* clstack,
                  double* corrstack, double* eqmap) {
     if (K < 0) return CLM_E_ARG;
-    const size_t KK = (size_t)K * K;
-    if (clstack) std::memset(clstack, 0, KK * sizeof(double));
-    if (corrstack) std::memset(corrstack, 0, KK * sizeof(double));
-    if (eqmap) std::memset(eqmap, 0, KK * sizeof(double));
-    long long e = 0;
-    for (long long k1 = 0; k1 < K; k1++)
-        for (long long k2 = k1 + 1; k2 < K; k2++, e++) {
-            const size_t a = (size_t)(k1 + k2 * K), b = (size_t)(k2 + k1 * K);  // (k1,k2), (k2,k1)
-            if (clstack) {
-                clstack[a] = cl1 ? cl1[e] : 0;
-                clstack[b] = cl2 ? cl2[e] : 0;
+    // one column of the K x K column-major outputs per step, columns split
+    // over host threads: entry (r, c) is pair (r, c) above the diagonal
+    // (cl1, 1 - corr, its row number), pair (c, r) below it (cl2, 0, 0)
+    host_parallel(K, 65536 / std::max(K, 1) + 1, [&](long long c0, long long c1) {
+        for (long long c = c0; c < c1; c++) {
+            const size_t col = (size_t)c * K;
+            for (long long r = 0; r < K; r++) {
+                double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+                if (r < c) {
+                    const long long e = pair_slot(r, c, K);
+                    v1 = cl1 ? cl1[e] : 0;
+                    v2 = corr && corr[e] != 0.0 ? 1.0 - corr[e] : 0.0;  // :764
+                    v3 = (double)(e + 1);
+                } else if (r > c) {
+                    v1 = cl2 ? cl2[pair_slot(c, r, K)] : 0;
+                }
+                if (clstack) clstack[col + r] = v1;
+                if (corrstack) corrstack[col + r] = v2;
+                if (eqmap) eqmap[col + r] = v3;
             }
-            if (corrstack && corr) corrstack[a] = corr[e] != 0.0 ? 1.0 - corr[e] : 0.0;  // :764
-            if (eqmap) eqmap[a] = (double)(e + 1);
         }
+    });
     return CLM_OK;
 }
 
@@ -1623,43 +1658,77 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
         out[3] = -cb[b];
     };
     const long long P = clm_num_pairs(K);
-    auto slot = [&](long long k1, long long k2) { return k1 * K - k1 * (k1 + 1) / 2 + (k2 - k1 - 1); };
-    const bool fill_pass = ir != nullptr && pr != nullptr;
-    long long n = 0;
-    if (jc) jc[0] = 0;
-    for (long long k = 0; k < K; k++) {
-        for (int xy = 0; xy < 2; xy++) {
-            for (long long j = 0; j < k; j++) {  // rows where k is the second image
-                const long long e = slot(j, k);
-                double c4[4];
-                coefs(e, c4);
-                const double v = c4[2 + xy];
-                if (v != 0.0) {
-                    if (fill_pass) { ir[n] = e; pr[n] = v; }
-                    n++;
-                }
+    const long long ncol = 2 * (long long)K;
+    // column q = 2k + xy (image k, coordinate xy): rows where k is the second
+    // image (pairs (j, k), j < k), then rows where it is the first (pairs
+    // (k, m), m > k), nonzero coefficients only (MATLAB's sparse keeps no
+    // zeros); column 2K: the right-hand side.  Columns are counted, then
+    // filled, by host threads; the result does not depend on their number.
+    auto column = [&](long long q, int64_t* out_ir, double* out_pr) -> long long {
+        const long long k = q >> 1;
+        const int xy = (int)(q & 1);
+        long long n = 0;
+        double c4[4];
+        for (long long j = 0; j < k; j++) {
+            const long long e = pair_slot(j, k, K);
+            coefs(e, c4);
+            const double v = c4[2 + xy];
+            if (v != 0.0) {
+                if (out_ir) { out_ir[n] = e; out_pr[n] = v; }
+                n++;
             }
-            for (long long m = k + 1; m < K; m++) {  // rows where k is the first image
-                const long long e = slot(k, m);
-                double c4[4];
-                coefs(e, c4);
-                const double v = c4[xy];
-                if (v != 0.0) {
-                    if (fill_pass) { ir[n] = e; pr[n] = v; }
-                    n++;
-                }
+        }
+        for (long long m = k + 1; m < K; m++) {
+            const long long e = pair_slot(k, m, K);
+            coefs(e, c4);
+            const double v = c4[xy];
+            if (v != 0.0) {
+                if (out_ir) { out_ir[n] = e; out_pr[n] = v; }
+                n++;
             }
-            if (jc) jc[2 * k + xy + 1] = n;
         }
-    }
-    for (long long e = 0; e < P; e++) {  // last column: right-hand side
-        if (shift1d[e] != 0.0) {
-            if (fill_pass) { ir[n] = e; pr[n] = shift1d[e]; }
-            n++;
+        return n;
+    };
+    std::vector<long long> cnt(ncol + 1, 0);
+    host_parallel(ncol, 65536 / std::max(K, 1) + 1, [&](long long q0, long long q1) {
+        for (long long q = q0; q < q1; q++) cnt[q] = column(q, nullptr, nullptr);
+    });
+    // right-hand side, counted in blocks
+    const long long nblk = std::min<long long>(std::max<long long>(P / 65536, 1), 4096);
+    std::vector<long long> rcnt(nblk, 0);
+    host_parallel(nblk, 1, [&](long long b0, long long b1) {
+        for (long long b = b0; b < b1; b++) {
+            long long n = 0;
+            for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++) n += shift1d[e] != 0.0;
+            rcnt[b] = n;
         }
+    });
+    std::vector<long long> start(ncol + 2, 0);
+    for (long long q = 0; q < ncol; q++) start[q + 1] = start[q] + cnt[q];
+    long long rtot = 0;
+    for (long long b = 0; b < nblk; b++) rtot += rcnt[b];
+    start[ncol + 1] = start[ncol] + rtot;
+    if (jc)
+        for (long long q = 0; q <= ncol + 1; q++) jc[q] = start[q];
+    if (nnz) *nnz = start[ncol + 1];
+    if (ir && pr) {
+        host_parallel(ncol, 65536 / std::max(K, 1) + 1, [&](long long q0, long long q1) {
+            for (long long q = q0; q < q1; q++) column(q, ir + start[q], pr + start[q]);
+        });
+        std::vector<long long> rstart(nblk, start[ncol]);
+        for (long long b = 1; b < nblk; b++) rstart[b] = rstart[b - 1] + rcnt[b - 1];
+        host_parallel(nblk, 1, [&](long long b0, long long b1) {
+            for (long long b = b0; b < b1; b++) {
+                long long n = rstart[b];
+                for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++)
+                    if (shift1d[e] != 0.0) {
+                        ir[n] = e;
+                        pr[n] = shift1d[e];
+                        n++;
+                    }
+            }
+        });
     }
-    if (jc) jc[2 * (long long)K + 1] = n;
-    if (nnz) *nnz = n;
     return CLM_OK;
 }
 
