@@ -259,6 +259,10 @@ struct Plan {
     long long tw_max = 0;        // largest group refined by a twin (pairs)
     long long P = 0;
     int n_unc_cap = 0;           // entries of the shared pass-2 buffers
+    // FP32 twins built only when their group has uncertain pairs (runs with
+    // a host wait between the phases; a captured job builds them up front)
+    bool twin_lazy = false;
+    bool twins_built = false;
 };
 
 int enqueue_fill(clm_ctx* c, void* p, long long n, int v) {
@@ -437,14 +441,17 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
         const long long K = g.K;
         const size_t osz = j.opk == OPK_F32 ? 4 : 2;
         if (j.twin_src >= 0) {
-            // FP32 twin: own operand planes and A-stacks, everything else shared
-            if ((rc = ensure(c, B.Bop, (size_t)K * 2 * g.Lp * g.Rp * 4))) return rc;
-            if ((rc = ensure(c, B.astk, (size_t)K * g.NAT * (32 * 2 * (size_t)g.Rp * 4)))) return rc;
+            // FP32 twin: own operand planes and A-stacks (allocated here, or
+            // by build_twins when they are first needed), everything else shared
+            if (!pl.twin_lazy) {
+                if ((rc = ensure(c, B.Bop, (size_t)K * 2 * g.Lp * g.Rp * 4))) return rc;
+                if ((rc = ensure(c, B.astk, (size_t)K * g.NAT * (32 * 2 * (size_t)g.Rp * 4)))) return rc;
+            }
             StackDev& d = pl.table[ti];
             d = pl.table[j.twin_src];
             d.g = g;
-            d.astk = B.astk.p;
-            d.Bop = B.Bop.p;
+            d.astk = pl.twin_lazy ? nullptr : B.astk.p;
+            d.Bop = pl.twin_lazy ? nullptr : B.Bop.p;
             d.coef = (2.0f * 0x1p-24f + 0x1p-48f + (2.0f * g.Rp + 16.0f) * 0x1p-24f) * 1.01f;
             d.sub = 0.0f;
             d.tau_t = j.force64 ? (float)(0.5 * j.p.tie_margin) : 0.0f;
@@ -582,6 +589,96 @@ int prepare(clm_ctx* c, Plan& pl, int n, const clm_params* ps, const clm_stack_i
 
 // Per-stack preprocessing (lines, filter, normalization, operand planes) and
 // A-stacks.
+// An FP32 twin's operand planes and A-stacks, from its source stack's P32
+// (which the source's prep has written): A-stacks of all images, or of the
+// nimg images listed in imgs (device) in that order.
+int enqueue_twin_prep(clm_ctx* c, const Plan& pl, size_t ti, const int* imgs = nullptr, int nimg = -1) {
+    const StackJob& j = pl.jobs[pl.order[ti]];
+    const StackBufs& B = c->sb[pl.sb_base + ti];
+    const Geom& g = j.s.g;
+    const StackDev& src = pl.table[j.twin_src];
+    if (nimg < 0) nimg = g.K;
+    const long long nb32 = (long long)g.K * 2 * g.Lp * g.Rp;
+    hipLaunchKernelGGL(bop32_kernel, dim3((unsigned)((nb32 + 255) / 256)), dim3(256), 0, c->stream, g, src.P32,
+                       (float*)B.Bop.p);
+    HIPCHK(c, hipGetLastError());
+    const long long nch = (long long)nimg * g.NAT * (long long)(32 * 2 * (size_t)g.Rp * 4 / 16);
+    AstkFn af = pick_astk(OPK_F32, g.Rp, g.sym != 0);
+    if (!af) return fail(c, CLM_E_UNSUP, "no FP32 A-stack kernel for Rp=%d", g.Rp);
+    if (nch > 0) {
+        hipLaunchKernelGGL(af, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, c->stream, g, src.P32, src.ph32,
+                           (int4*)B.astk.p, imgs, nimg);
+        HIPCHK(c, hipGetLastError());
+    }
+    return CLM_OK;
+}
+
+// Lazy twins, once some group turned out to need them: the images k1 of the
+// uncertain pairs (the twins re-run only those pairs, whose A side is k1)
+// get FP32 A-stacks, the device table maps every image to its A-stack
+// (-1: none), and the twins' operand planes are built for all images.
+// ctr: the counters as read back after the re-score.
+int build_twins(clm_ctx* c, Plan& pl, const std::vector<unsigned long long>& ctr) {
+    int rc;
+    // uncertain pairs of every group refined by a twin, by table index
+    std::vector<std::vector<int>> k1s(pl.order.size());
+    for (size_t gi = 0; gi < pl.groups.size(); gi++) {
+        const Group& G = pl.groups[gi];
+        if (G.twin < 0) continue;
+        const int n = (int)(unsigned)(ctr[16 + GSTRIDE * gi + 1] & 0xffffffffu);
+        if (n <= 0) continue;
+        std::vector<SlotRef> ul((size_t)std::min<long long>(n, G.P));
+        HIPCHK(c, hipMemcpy(ul.data(), (const SlotRef*)c->ulist.p + G.ubase, ul.size() * sizeof(SlotRef),
+                            hipMemcpyDeviceToHost));
+        for (const SlotRef& r : ul) {
+            const long long K = pl.table[r.stack].g.K;
+            // slot -> k1 (pairs k1 < k2 in row-major order)
+            long long k1 = (long long)std::floor(((2.0 * K - 1.0) -
+                                                  std::sqrt((2.0 * K - 1.0) * (2.0 * K - 1.0) - 8.0 * (double)r.slot)) / 2.0);
+            k1 = std::max(0LL, std::min(k1, K - 2));
+            while (k1 > 0 && pair_slot(k1, k1 + 1, K) > r.slot) k1--;
+            while (k1 + 1 < K - 1 && pair_slot(k1 + 1, k1 + 2, K) <= r.slot) k1++;
+            k1s[r.stack].push_back((int)k1);
+        }
+    }
+    std::vector<std::vector<int>> hmaps(pl.order.size());
+    for (size_t ti = 0; ti < pl.order.size(); ti++) {
+        const StackJob& j = pl.jobs[pl.order[ti]];
+        if (j.twin_src < 0) continue;
+        StackBufs& B = c->sb[pl.sb_base + ti];
+        const Geom& g = j.s.g;
+        const long long K = g.K;
+        // image -> A-stack index, then the images in index order
+        std::vector<int>& h = hmaps[ti];
+        h.assign((size_t)K, -1);
+        std::vector<int> imgs;
+        for (int k : k1s[j.twin_src])
+            if (h[k] < 0) {
+                h[k] = (int)imgs.size();
+                imgs.push_back(k);
+            }
+        h.insert(h.end(), imgs.begin(), imgs.end());
+        const int nimg = (int)imgs.size();
+        if ((rc = ensure(c, B.Bop, (size_t)K * 2 * g.Lp * g.Rp * 4))) return rc;
+        if ((rc = ensure(c, B.astk, (size_t)std::max(nimg, 1) * g.NAT * (32 * 2 * (size_t)g.Rp * 4)))) return rc;
+        if ((rc = upload(c, B.amap, h))) return rc;
+        pl.table[ti].astk = B.astk.p;
+        pl.table[ti].Bop = B.Bop.p;
+        pl.table[ti].amap = (const int*)B.amap.p;
+    }
+    if ((rc = upload(c, c->table, pl.table))) return rc;
+    pl.twins_built = true;
+    for (size_t ti = 0; ti < pl.order.size(); ti++) {
+        if (pl.jobs[pl.order[ti]].twin_src < 0) continue;
+        const long long K = pl.jobs[pl.order[ti]].s.g.K;
+        const int nimg = (int)(hmaps[ti].size() - (size_t)K);
+        if ((rc = enqueue_twin_prep(c, pl, ti, (const int*)c->sb[pl.sb_base + ti].amap.p + K, nimg))) return rc;
+    }
+    // the host maps are released on return: the uploads must be done
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
 int enqueue_prep(clm_ctx* c, const Plan& pl) {
     int rc;
     unsigned long long* ctr = (unsigned long long*)c->ctr.p;
@@ -590,17 +687,7 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
         const StackBufs& B = c->sb[pl.sb_base + ti];
         const Geom& g = j.s.g;
         if (j.twin_src >= 0) {
-            const StackDev& src = pl.table[j.twin_src];
-            const long long nb32 = (long long)g.K * 2 * g.Lp * g.Rp;
-            hipLaunchKernelGGL(bop32_kernel, dim3((unsigned)((nb32 + 255) / 256)), dim3(256), 0, c->stream, g,
-                               src.P32, (float*)B.Bop.p);
-            HIPCHK(c, hipGetLastError());
-            const long long nch = (long long)g.K * g.NAT * (long long)(32 * 2 * (size_t)g.Rp * 4 / 16);
-            AstkFn af = pick_astk(OPK_F32, g.Rp, g.sym != 0);
-            if (!af) return fail(c, CLM_E_UNSUP, "no FP32 A-stack kernel for Rp=%d", g.Rp);
-            hipLaunchKernelGGL(af, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, c->stream, g, src.P32,
-                               src.ph32, (int4*)B.astk.p);
-            HIPCHK(c, hipGetLastError());
+            if (!pl.twin_lazy && (rc = enqueue_twin_prep(c, pl, ti))) return rc;
             continue;
         }
         if ((rc = enqueue_fill(c, B.nrm.p, g.K, 0))) return rc;
@@ -621,7 +708,7 @@ int enqueue_prep(clm_ctx* c, const Plan& pl) {
             AstkFn af = pick_astk(j.opk, g.Rp, g.sym != 0);
             if (!af) return fail(c, CLM_E_UNSUP, "no A-stack kernel for Rp=%d", g.Rp);
             hipLaunchKernelGGL(af, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, c->stream, g,
-                               (const float2*)B.P32.p, (const float2*)B.ph32.p, (int4*)B.astk.p);
+                               (const float2*)B.P32.p, (const float2*)B.ph32.p, (int4*)B.astk.p, nullptr, g.K);
             HIPCHK(c, hipGetLastError());
         }
     }
@@ -916,10 +1003,29 @@ void timed_stats(clm_ctx* c, clm_stats* st) {
     st->ms_total = event_ms(c->ev[0], c->ev[3]);
 }
 
-// Run a plan to completion (no graph).
-int run_plan(clm_ctx* c, const Plan& pl, bool timed, clm_stats* st) {
+// Run a plan to completion (no graph).  With lazy twins the host waits
+// after the re-score: the twins are built only if a group refined by them
+// has uncertain pairs (else their refinement launches find nothing to do).
+int run_plan(clm_ctx* c, Plan& pl, bool timed, clm_stats* st) {
     int rc;
-    if ((rc = enqueue_job(c, pl, timed))) return rc;
+    if (!pl.twin_lazy) {
+        if ((rc = enqueue_job(c, pl, timed))) return rc;
+    } else {
+        if ((rc = enqueue_job(c, pl, timed, PHASE_A))) return rc;
+        bool any = false;
+        for (const Group& G : pl.groups) any = any || G.twin >= 0;
+        if (any) {
+            std::vector<unsigned long long> h(16 + GSTRIDE * pl.groups.size());
+            HIPCHK(c, hipMemcpyAsync(h.data(), c->ctr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            bool need = false;
+            for (size_t gi = 0; gi < pl.groups.size(); gi++)
+                if (pl.groups[gi].twin >= 0 && (int)(unsigned)(h[16 + GSTRIDE * gi + 1] & 0xffffffffu) > 0) need = true;
+            if (need && (rc = build_twins(c, pl, h))) return rc;
+        }
+        if ((rc = enqueue_job(c, pl, timed, PHASE_B))) return rc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return add_stats(c, pl, st);
 }
@@ -967,6 +1073,10 @@ std::vector<unsigned long long> signature(const clm_ctx* c, const Plan& pl) {
 int run_job_core(clm_ctx* c, int n, const clm_params* ps, const clm_stack_io* ios, bool graph, clm_stats* st,
                  const StackJob* extra) {
     Plan pl;
+    // lazy twins need a host wait between the phases, and grow memory after
+    // kernels have run: not in a captured job, nor in an allocation-free call
+    // (which checks every buffer against its scratch array before any kernel)
+    pl.twin_lazy = !graph && !c->arena;
     int rc = prepare(c, pl, n, ps, ios, true, extra);
     if (rc) return rc;
     begin_stats(pl, st);
@@ -1114,6 +1224,7 @@ int run_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float*
                            (double*)c->w_corr.p, (double*)c->w_shift.p};
     {
         Plan plw;
+        plw.twin_lazy = !c->arena;
         if ((rc = prepare(c, plw, 1, &pw, &iow, true, &xw, 0))) return rc;
         if ((rc = run_plan(c, plw, false, nullptr))) return rc;
         hipLaunchKernelGGL(window_merge_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream, (int)K,
@@ -1144,6 +1255,7 @@ int run_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float*
     xf.sel_ntiles = tot + 1;
     {
         Plan plf;
+        plf.twin_lazy = !c->arena;
         if ((rc = prepare(c, plf, 1, p, &out, true, &xf, 2))) return rc;
         if (plf.jobs[0].v.W / plf.jobs[0].v.CPP != ppw) return fail(c, CLM_E_HIP, "fallback tile shape changed");
         if ((rc = run_plan(c, plf, false, nullptr))) return rc;
@@ -1408,7 +1520,8 @@ void clm_destroy(clm_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (StackBufs& B : c->sb) {
-        DevBuf* sbufs[] = {&B.P64, &B.P32, &B.Bop, &B.nrm, &B.astk, &B.recs, &B.ph32, &B.ph64, &B.shiftv, &B.H, &B.W};
+        DevBuf* sbufs[] = {&B.P64, &B.P32, &B.Bop, &B.nrm, &B.astk, &B.recs, &B.ph32, &B.ph64, &B.shiftv, &B.H, &B.W,
+                           &B.amap};
         for (DevBuf* b : sbufs) release(*b);
     }
     for (TileCache& tc : c->tcache) release(tc.tiles);

@@ -21,6 +21,7 @@ struct DevBuf {
 
 struct StackBufs {
     DevBuf P64, P32, Bop, nrm, astk, recs, ph32, ph64, shiftv, H, W;
+    DevBuf amap;  // a twin's image -> A-stack index map (K ints), then its image list
 };
 
 // Pass-1 tiles of one group, kept while the group's shapes repeat.

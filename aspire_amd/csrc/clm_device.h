@@ -50,7 +50,9 @@ struct Geom {
 // list entries name their stack.
 struct StackDev {
     Geom g;
-    const void* astk;         // [K][NAT] A-tiles of the operand type
+    const void* astk;         // [K][NAT] A-tiles of the operand type (or [n][NAT], see amap)
+    const int* amap;          // null, or image -> its A-tiles' index in astk (A-stacks built
+                              // for some images only: an FP32 twin's, for its uncertain pairs)
     const void* Bop;          // [K][2][Lp][Rp] operand planes
     const float* nrm;         // [K] largest half-line norm per image
     PairRec* recs;            // [P] pass-1 records

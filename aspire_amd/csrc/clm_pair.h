@@ -55,19 +55,21 @@ template <typename OpT, int NK> struct ATile {
     __device__ static inline int phys(int row, int c) { return (c + row) % CPR; }
 };
 
+// imgs: null (A-stacks of images 0..nimg-1 in order) or the nimg images
+// whose A-stacks are built, in this order.
 template <typename OpT, int NK, bool SYM>
 __global__ void astack_kernel(Geom g, const float2* __restrict__ P32, const float2* __restrict__ ph32,
-                              int4* __restrict__ astk) {
+                              int4* __restrict__ astk, const int* __restrict__ imgs, int nimg) {
     using AT = ATile<OpT, NK>;
     constexpr int Rp = AT::Rp, CPR = AT::CPR, EPC = AT::EPC;
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long)g.K * g.NAT * AT::NCH) return;
+    if (q >= (long)nimg * g.NAT * AT::NCH) return;
     const int pch = (int)(q % CPR);
     long r = q / CPR;
     const int row = (int)(r % 32);
     r /= 32;
     const int at = (int)(r % g.NAT);
-    const int k = (int)(r / g.NAT);
+    const int k = imgs ? imgs[r / g.NAT] : (int)(r / g.NAT);
     const int c = (pch - row % CPR + CPR) % CPR;   // logical chunk stored here
     const bool imag = c >= CPR / 2;
     const int k0 = (c % (CPR / 2)) * EPC;
@@ -225,7 +227,8 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
         // A-tiles of image k1 stream through two LDS buffers: each thread
         // loads its 16-byte chunks of tile at+1 into registers one step
         // ahead and writes them to LDS at the start of the next step.
-        const int4* asrc = (const int4*)((const char*)sd.astk + (size_t)k1 * g.NAT * AT::BYTES);
+        const int ka = sd.amap ? sd.amap[k1] : k1;
+        const int4* asrc = (const int4*)((const char*)sd.astk + (size_t)ka * g.NAT * AT::BYTES);
         // native vectors: copies of HIP's int4 wrapper kept this buffer in
         // private memory
         typedef int v4i __attribute__((ext_vector_type(4)));
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(512) void pair_kernel(PairArgs a) {
 }
 
 using PairFn = void (*)(PairArgs);
-using AstkFn = void (*)(Geom, const float2*, const float2*, int4*);
+using AstkFn = void (*)(Geom, const float2*, const float2*, int4*, const int*, int);
 
 // Instantiations, one set per (operand type, grid symmetry): ct = column
 // tiles per wave, NK = k-steps of the padded ray (Rp / k-step), emit = second

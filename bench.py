@@ -752,11 +752,30 @@ def vote_bench(args):
            "config": {"workload": "voting synchronization matrix, K=5000, n_theta=360", "K": K, "pairs": P},
            "stats": st, "pairs_with_line": ntri,
            "exp_per_s_upper_bound": 60.0 * P * (K - 2) / sec,
-           "roofline": None,
+           "roofline": vote_roofline(K, st),
            "cpu_baseline": {"value": sample.size / cpu_sec, "unit": "pairs/s", "cores": 1, "kind": "port",
                             "sample": f"{sample.size} seeded pairs, FP64 C oracle (one thread)"},
            "parity_sample": {"pairs": int(sample.size), "blocks_equal_1e-10": same}}
     print(json.dumps(out))
+
+
+def vote_roofline(K, st):
+    """FP64 roofline of the vote at K=5000: its FP64 operations per vote
+    as rocprof counts them (ADD + MUL + 2 FMA + TRANS wave instructions x 64,
+    profiles/vote_kernel_k5000_f64.md: the reference's arithmetic, each
+    triangle evaluated once, 60 Gaussians of the histogram per triangle)
+    over the vote's own device time (HIP events around its kernels)."""
+    if K != 5000:
+        return None
+    flops = 6.9e13
+    ms = float(st.get("ms_vote", 0.0))
+    if ms <= 0:
+        return None
+    achieved = flops / (ms * 1e-3)
+    return {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+            "frac": achieved / 78.6e12, "traffic": None, "kernel": "vote (peak_kernel + rot_kernel)",
+            "kernel_ms": ms, "flops_per_vote": flops,
+            "flops_basis": "FP64 operations counted by rocprof for this workload (each triangle once)"}
 
 
 def sync3n_bench(args):
