@@ -93,13 +93,15 @@ __device__ inline double2 trig(const double2* cs, int nt, int d) {
     return cs[i < 0 ? 0 : (i > 2 * nt ? 2 * nt : i)];
 }
 
-// Angle (degrees) of triangle (k1, k2, k3), NaN when k3 takes no part.
-__device__ inline double vote_angle(const unsigned short* r1, const unsigned short* r2, const unsigned short* c1_,
-                                    const unsigned short* c2_, int k1, int k3, int l12, int l21,
-                                    const double2* cs, int nt) {
-    const int l13 = r1[k3], l23 = r2[k3];
+// Cosine of the angle of triangle (k1, k2, k3), clamped to [-1, 1]; NaN
+// when k3 takes no part.  The clstack rows and columns of k1 and k2 are read
+// at index i (i = k3, or k3's place in a staged chunk of them).
+__device__ inline double vote_cos(const unsigned short* r1, const unsigned short* r2, const unsigned short* c1_,
+                                  const unsigned short* c2_, int i, int k1, int k3, int l12, int l21,
+                                  const double2* cs, int nt) {
+    const int l13 = r1[i], l23 = r2[i];
     if (k3 == k1 || l13 == 0 || l23 == 0) return NAN;
-    const int l31 = c1_[k3], l32 = c2_[k3];
+    const int l31 = c1_[i], l32 = c2_[i];
     // theta1 = (l13 - l12) 2pi/L, theta2 = (l21 - l23) 2pi/L, theta3 = (l32 - l31) 2pi/L
     const double2 t1 = trig(cs, nt, l13 - l12), t2 = trig(cs, nt, l21 - l23), t3 = trig(cs, nt, l32 - l31);
     const double c1 = t1.x, c2 = t2.x, c3 = t3.x;
@@ -107,7 +109,25 @@ __device__ inline double vote_angle(const unsigned short* r1, const unsigned sho
     if (!(cond > 1.0e-5)) return NAN;
     double cp = (c3 - c1 * c2) / (t1.y * t2.y);
     cp = cp > 1 ? 1 : (cp < -1 ? -1 : cp);
-    return acos(cp) * 180 / M_PI;
+    return cp;
+}
+
+// Angle (degrees) of triangle (k1, k2, k3), NaN when k3 takes no part.
+__device__ inline double vote_angle(const unsigned short* r1, const unsigned short* r2, const unsigned short* c1_,
+                                    const unsigned short* c2_, int k1, int k3, int l12, int l21,
+                                    const double2* cs, int nt) {
+    const double cp = vote_cos(r1, r2, c1_, c2_, k3, k1, k3, l12, l21, cs, nt);
+    return isnan(cp) ? cp : acos(cp) * 180 / M_PI;
+}
+
+// The (cos, sin) table in LDS when it fits (dynamic LDS of the launch), the
+// same values read from global memory otherwise.
+__device__ inline const double2* stage_trig(const double2* cs, int nt, int use_lds) {
+    extern __shared__ double2 s_cs[];
+    if (!use_lds) return cs;
+    for (int i = threadIdx.x; i <= 2 * nt; i += blockDim.x) s_cs[i] = cs[i];
+    __syncthreads();
+    return s_cs;
 }
 
 // Pair (k1, k2) of flat index e (k1 < k2, row-major over k1).
@@ -127,9 +147,11 @@ __device__ inline void pair_of(long long e, int K, int& k1, int& k2) {
 // many waves per SIMD: the loop is a chain of dependent FP64 operations per
 // angle, and occupancy hides its latency.
 __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const unsigned short* M,
-                                                   const unsigned short* MT, const double2* cs, int* peak) {
+                                                   const unsigned short* MT, const double2* cs_g, int* peak,
+                                                   int cs_lds) {
     __shared__ double s_h[4][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double2* cs = stage_trig(cs_g, n_theta, cs_lds);
     const long long P = (long long)K * (K - 1) / 2;
     for (long long e = blockIdx.x; e < P; e += gridDim.x) {
         int k1, k2;
@@ -145,21 +167,42 @@ __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const uns
         }
         double h = 0.0;
         const double x = lane < NT ? tick(lane) : 0.0;
+        // lane q's angle to every lane through scalar registers
+        auto angle_of = [](int alo, int ahi, int q) {
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane(alo, q);
+            const long long hi = (long long)__builtin_amdgcn_readlane(ahi, q);
+            return __longlong_as_double((hi << 32) | lo);
+        };
+        auto gauss = [&](double a) {
+            double t = 2 * a * x;
+            t = t - a * a;
+            t = t - x * x;
+            return exp(t / 18.0);
+        };
         for (int base = 64 * wave; base < K; base += 256) {
             const int k3 = base + lane;
             const double ang = k3 < K ? vote_angle(r1, r2, c1, c2, k1, k3, l12, l21, cs, n_theta) : NAN;
-            // lane q's angle to every lane through scalar registers
             const long long ab = __double_as_longlong(ang);
             const int alo = (int)ab, ahi = (int)(ab >> 32);
-            for (int q = 0; q < 64; q++) {
-                const unsigned lo = (unsigned)__builtin_amdgcn_readlane(alo, q);
-                const long long hi = (long long)__builtin_amdgcn_readlane(ahi, q);
-                const double a = __longlong_as_double((hi << 32) | lo);
-                if (isnan(a)) continue;  // uniform across the wave
-                double t = 2 * a * x;
-                t = t - a * a;
-                t = t - x * x;
-                h = h + exp(t / 18.0);
+            // the lanes with an angle (others take no part), in lane order;
+            // UNR independent exponentials per step, added in that order
+            constexpr int UNR = 4;
+            unsigned long long m = __ballot(!isnan(ang));
+            while (__popcll(m) >= UNR) {
+                double e[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    const int q = __builtin_ctzll(m);
+                    m &= m - 1;
+                    e[u] = gauss(angle_of(alo, ahi, q));
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; u++) h = h + e[u];
+            }
+            while (m) {
+                const int q = __builtin_ctzll(m);
+                m &= m - 1;
+                h = h + gauss(angle_of(alo, ahi, q));
             }
         }
         s_h[wave][lane] = h;
@@ -180,14 +223,45 @@ __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const uns
     }
 }
 
+// Sum over the 64 lanes in a fixed order, the same in every lane: DPP adds
+// within each row of 16 lanes, then the four row sums in row order.  Every
+// lane of the wave must be active.
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+    const unsigned long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffu), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double readlane_d(double x, int l) {
+    const unsigned long long u = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double wave_sum(double x) {
+    x += dpp_d<0xb1>(x);
+    x += dpp_d<0x4e>(x);
+    x += dpp_d<0x124>(x);
+    x += dpp_d<0x128>(x);
+    return ((readlane_d(x, 0) + readlane_d(x, 16)) + readlane_d(x, 32)) + readlane_d(x, 48);
+}
+
+// Third images per staged chunk of the rotation kernel (4 x 8 KB of LDS).
+constexpr int RCH = 4096;
+
 // Triangles within 6 degrees of each pair's peak: their relative rotations,
-// averaged into the pair's 2x2 block of S and its transpose.
+// averaged into the pair's 2x2 block of S and its transpose.  The pair's
+// clstack rows and columns are staged in LDS a chunk of third images at a
+// time (independent loads of the whole chunk, then LDS reads in the loop).
 __global__ __launch_bounds__(256) void rot_kernel(int K, int n_theta, const unsigned short* M,
-                                                  const unsigned short* MT, const double2* cs, const int* peak,
-                                                  double* S) {
-    __shared__ double s_red[4][256];
-    __shared__ int s_cnt[256];
+                                                  const unsigned short* MT, const double2* cs_g, const int* peak,
+                                                  double* S, int cs_lds) {
+    __shared__ double s_red[4][4];
+    __shared__ int s_cnt[4];
+    __shared__ unsigned short s_rc[4][RCH];  // r1, r2, c1, c2 of the chunk
     const int tid = threadIdx.x;
+    const double2* cs = stage_trig(cs_g, n_theta, cs_lds);
     const long long P = (long long)K * (K - 1) / 2;
     const double L = (double)n_theta;
     const size_t n2 = 2 * (size_t)K;
@@ -208,46 +282,75 @@ __global__ __launch_bounds__(256) void rot_kernel(int K, int n_theta, const unsi
             const double aa = (l12 - 1) * 2 * M_PI / L, bb = (l21 - 1) * 2 * M_PI / L;
             const double ang1 = M_PI - bb, ang3 = aa - M_PI;
             const double sa = sin(ang1), ca = cos(ang1), sc = sin(ang3), cc = cos(ang3);
-            for (int k3 = tid; k3 < K; k3 += 256) {
-                const double ang = vote_angle(r1, r2, c1, c2, k1, k3, il12, il21, cs, n_theta);
-                if (isnan(ang) || !(fabs(ang - xp) < 360.0 / NT)) continue;
-                cnt++;
-                const int d1 = (int)c2[k3] - (int)c1[k3];  // cl(k3,k2) - cl(k3,k1)
-                const int d2 = (int)r2[k3] - il21;         // cl(k2,k3) - cl(k2,k1)
-                const int d3 = (int)r1[k3] - il12;         // cl(k1,k3) - cl(k1,k2)
-                const double idx2 = d2, idx3 = d3;
-                // cos(2 pi idx / L) of the three differences
-                const double a = trig(cs, n_theta, d1).x;
-                const double b = trig(cs, n_theta, d2).x;
-                const double c = trig(cs, n_theta, d3).x;
-                const double cond = 1 + 2 * a * b * c - (a * a + b * b + c * c);
-                if (!(cond > 1.0e-5)) continue;
-                double ca_ = (a - b * c) / sqrt(1 - b * b) / sqrt(1 - c * c);
-                const double TOL_idx = 1e-12;
-                const bool ind1 = (idx3 > L / 2 + TOL_idx) || (idx3 < -TOL_idx && idx3 > -L / 2);
-                const bool ind2 = (idx2 > L / 2 + TOL_idx) || (idx2 < -TOL_idx && idx2 > -L / 2);
-                if (ind1 != ind2) ca_ = -ca_;
-                ca_ = ca_ > 1 ? 1 : (ca_ < -1 ? -1 : ca_);
-                const double cb = cos(acos(ca_));
-                r00 = r00 + (cc * ca - sc * cb * sa);
-                r01 = r01 + (-cc * sa - sc * cb * ca);
-                r10 = r10 + (sc * ca + cc * cb * sa);
-                r11 = r11 + (-sa * sc + cc * cb * ca);
-            }
-            s_red[0][tid] = r00;
-            s_red[1][tid] = r01;
-            s_red[2][tid] = r10;
-            s_red[3][tid] = r11;
-            s_cnt[tid] = cnt;
-            __syncthreads();
-            for (int s = 128; s > 0; s >>= 1) {
-                if (tid < s) {
-#pragma unroll
-                    for (int m = 0; m < 4; m++) s_red[m][tid] += s_red[m][tid + s];
-                    s_cnt[tid] += s_cnt[tid + s];
+            // cosines of angles certainly farther than 360/NT degrees (by
+            // 1e-4 degree, far beyond the rounding of acos and cos) from the
+            // peak: the triangle is out without its acos; the rest take the
+            // exact test
+            const double w = 360.0 / NT + 1e-4;
+            const double c_hi = xp - w > 0 ? cos((xp - w) * M_PI / 180) : INFINITY;
+            const double c_lo = xp + w < 180 ? cos((xp + w) * M_PI / 180) : -INFINITY;
+            for (int k0 = 0; k0 < K; k0 += RCH) {
+                const int nk = K - k0 < RCH ? K - k0 : RCH;
+                __syncthreads();  // the previous chunk's reads are done
+                for (int i = tid; i < nk; i += 256) {
+                    s_rc[0][i] = r1[k0 + i];
+                    s_rc[1][i] = r2[k0 + i];
+                    s_rc[2][i] = c1[k0 + i];
+                    s_rc[3][i] = c2[k0 + i];
                 }
                 __syncthreads();
+                for (int i = tid; i < nk; i += 256) {
+                    const int k3 = k0 + i;
+                    const double cp = vote_cos(s_rc[0], s_rc[1], s_rc[2], s_rc[3], i, k1, k3, il12, il21, cs, n_theta);
+                    if (isnan(cp) || cp >= c_hi || cp <= c_lo) continue;
+                    const double ang = acos(cp) * 180 / M_PI;
+                    if (!(fabs(ang - xp) < 360.0 / NT)) continue;
+                    cnt++;
+                    const int d1 = (int)s_rc[3][i] - (int)s_rc[2][i];  // cl(k3,k2) - cl(k3,k1)
+                    const int d2 = (int)s_rc[1][i] - il21;             // cl(k2,k3) - cl(k2,k1)
+                    const int d3 = (int)s_rc[0][i] - il12;             // cl(k1,k3) - cl(k1,k2)
+                    const double idx2 = d2, idx3 = d3;
+                    // cos(2 pi idx / L) of the three differences
+                    const double a = trig(cs, n_theta, d1).x;
+                    const double b = trig(cs, n_theta, d2).x;
+                    const double c = trig(cs, n_theta, d3).x;
+                    const double cond = 1 + 2 * a * b * c - (a * a + b * b + c * c);
+                    if (!(cond > 1.0e-5)) continue;
+                    double ca_ = (a - b * c) / sqrt(1 - b * b) / sqrt(1 - c * c);
+                    const double TOL_idx = 1e-12;
+                    const bool ind1 = (idx3 > L / 2 + TOL_idx) || (idx3 < -TOL_idx && idx3 > -L / 2);
+                    const bool ind2 = (idx2 > L / 2 + TOL_idx) || (idx2 < -TOL_idx && idx2 > -L / 2);
+                    if (ind1 != ind2) ca_ = -ca_;
+                    ca_ = ca_ > 1 ? 1 : (ca_ < -1 ? -1 : ca_);
+                    const double cb = cos(acos(ca_));
+                    r00 = r00 + (cc * ca - sc * cb * sa);
+                    r01 = r01 + (-cc * sa - sc * cb * ca);
+                    r10 = r10 + (sc * ca + cc * cb * sa);
+                    r11 = r11 + (-sa * sc + cc * cb * ca);
+                }
             }
+            // sums over the workgroup: each wave's (DPP within rows of 16,
+            // then the rows in order), then the four waves in order
+            const int lane = tid & 63, wave = tid >> 6;
+            r00 = wave_sum(r00);
+            r01 = wave_sum(r01);
+            r10 = wave_sum(r10);
+            r11 = wave_sum(r11);
+            const double wc = wave_sum((double)cnt);
+            if (lane == 0) {
+                s_red[0][wave] = r00;
+                s_red[1][wave] = r01;
+                s_red[2][wave] = r10;
+                s_red[3][wave] = r11;
+                s_cnt[wave] = (int)wc;
+            }
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) s_red[m][0] = ((s_red[m][0] + s_red[m][1]) + s_red[m][2]) + s_red[m][3];
+                s_cnt[0] = ((s_cnt[0] + s_cnt[1]) + s_cnt[2]) + s_cnt[3];
+            }
+            __syncthreads();
             const int n = s_cnt[0];
             if (n > 0) {
                 r00 = s_red[0][0] / n;
@@ -300,8 +403,11 @@ extern "C" int clm_syncmatrix_vote(clm_ctx* c, int32_t K, int32_t n_theta, const
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (P > 0) {
         const long long nb = std::min<long long>(P, (long long)c->n_cu * 16);
-        peak_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, cs, pk);
-        rot_kernel<<<(unsigned)nb, 256, 0, c->stream>>>(K, n_theta, M, MT, cs, pk, d_S);
+        // the (cos, sin) table in LDS up to 16 KB (n_theta <= 511)
+        const size_t cs_bytes = (size_t)(2 * n_theta + 1) * sizeof(double2);
+        const int cs_lds = cs_bytes <= 16384;
+        peak_kernel<<<(unsigned)nb, 256, cs_lds ? cs_bytes : 0, c->stream>>>(K, n_theta, M, MT, cs, pk, cs_lds);
+        rot_kernel<<<(unsigned)nb, 256, cs_lds ? cs_bytes : 0, c->stream>>>(K, n_theta, M, MT, cs, pk, d_S, cs_lds);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));

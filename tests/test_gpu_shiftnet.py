@@ -121,7 +121,8 @@ def window_offsets(prop, K, L, step):
     return np.rint(a / step) * step
 
 
-def test_windowed_search_properties(oracle, shifted):
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_windowed_search_properties(oracle, shifted, prec):
     """Window shifts are reference-grid shifts, so for every pair the window
     settles: (a) its corr is the FP64 value of a grid entry, (b) it never
     exceeds the full-grid maximum, (c) when the full-grid maximum lies in
@@ -129,14 +130,14 @@ def test_windowed_search_properties(oracle, shifted):
     window hands back get exactly the full-grid result."""
     import torch
 
-    say("case windowed-properties")
+    say("case windowed-properties", prec)
     pf = shifted.pf
     K, L = pf.shape[2], pf.shape[1] // 2
-    full, _ = clmatrix_records(pf, 12, 0.5, precision="bf16")
+    full, _ = clmatrix_records(pf, 12, 0.5, precision=prec)
     rng = np.random.default_rng(0)
     prop_np = (shifted.shifts + rng.normal(0, 0.3, size=shifted.shifts.shape)).astype(np.float32)
     prop = torch.as_tensor(prop_np).cuda()
-    rec, ws = clmatrix_windowed(to_dev(pf), prop, 12, 0.5, window=1.0, gate=0.0, precision="bf16")
+    rec, ws = clmatrix_windowed(to_dev(pf), prop, 12, 0.5, window=1.0, gate=0.0, precision=prec)
     say(ws)
     w = {k: v.cpu().numpy() for k, v in rec.items()}
     assert ws["window_shifts"] == 5 and ws["full_shifts"] == 49
