@@ -208,7 +208,7 @@ __device__ inline void f_both(double x, const double* prm, const double* lg, dou
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void tri_kernel(TriArgs a) {
+__device__ __forceinline__ void tri_body(TriArgs a) {
     __shared__ double Lp[9][KC][TB];
     __shared__ double Lq[9][KC][TB];
     __shared__ double Vp[MODE == SIGNS ? MAXE : 1][KC][TB];
@@ -361,6 +361,13 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
     if (l < n) p[l] = 0ull;
 }
 
+// One kernel per mode, for their own occupancy: the scores fit 128 VGPRs
+// (four waves per SIMD); the probabilities (log, exp, log1p) and the signs
+// product (staged vectors) need more and run at two.
+__global__ __launch_bounds__(256, 4) void tri_scores_kernel(TriArgs a) { tri_body<SCORES>(a); }
+__global__ __launch_bounds__(256) void tri_probs_kernel(TriArgs a) { tri_body<PROBS>(a); }
+__global__ __launch_bounds__(256) void tri_signs_kernel(TriArgs a) { tri_body<SIGNS>(a); }
+
 }  // namespace s3
 
 using namespace clmh;
@@ -421,7 +428,12 @@ int s3_finish(clm_ctx* c, const s3::TriArgs& a, clm_sync3n_stats* st) {
 template <int MODE>
 int s3_launch(clm_ctx* c, const s3::TriArgs& a, size_t dyn_lds) {
     const unsigned grid = (unsigned)std::min<long long>(a.ntiles, (long long)c->n_cu * 8);
-    s3::tri_kernel<MODE><<<grid, 256, dyn_lds, c->stream>>>(a);
+    if constexpr (MODE == s3::SCORES)
+        s3::tri_scores_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
+    else if constexpr (MODE == s3::PROBS)
+        s3::tri_probs_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
+    else
+        s3::tri_signs_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
     HIPCHK(c, hipGetLastError());
     return CLM_OK;
 }

@@ -869,7 +869,7 @@ def sync3n_bench(args):
            "pairs_probabilities": {"seconds": sec_p, "triangles_per_s": T / sec_p, "stats": stp},
            "signs_times_v": {"seconds": sec_v, "triangles_per_s": T / sec_v, "n_eigs": 4, "stats": stv},
            "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": 78.6, "unit": "TFLOP/s",
-                        "frac": achieved / 78.6e12, "traffic": traffic, "kernel": "tri_kernel<SCORES>",
+                        "frac": achieved / 78.6e12, "traffic": traffic, "kernel": "tri_scores_kernel",
                         "kernel_ms": st["ms_kernel"], "flops_per_triangle": 297,
                         "executed_flops_per_triangle": 659},
            "cpu_baseline": {"value": Ts / cpu_s, "unit": "triangles/s", "cores": int(nth), "kind": "port",
