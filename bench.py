@@ -282,7 +282,12 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": BF16_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / BF16_PEAK, "traffic": traffic,
                      "kernel": "pair_kernel (pass 1)", "kernel_ms": k_ms,
-                     "flops_per_pair": W_pair, "executed_flops_per_pair": W_pair / 2},
+                     "flops_per_pair": W_pair, "executed_flops_per_pair": W_pair / 2,
+                     "executed_frac": achieved / 2 / BF16_PEAK,
+                     "note": "flops_per_pair is the reference's arithmetic (8 S L^2 R); the kernel "
+                             "obtains it with half the products (X = Re'Re and Y = Im'Im give "
+                             "C1(s) = 2(X+Y) and C2(-s) = 2(X-Y) at once), so frac may pass 1; "
+                             "executed_frac is the MFMA work actually issued"},
         "cpu_baseline": cpu,
         "fp16": fp16,
         "fp32_gpu": {"value": fp32_value, "unit": "pairs/s", "K": Kf, "seconds": fp32_sec,
