@@ -35,6 +35,8 @@ EXPORTS = (
     "clm_clmatrix_filtered", "clm_clmatrix_filtered_device", "clm_syncmatrix_vote",
     "clm_set_log", "clm_clstack_mask",
     "clm_sync3n_triangles_scores", "clm_sync3n_pairs_probabilities", "clm_sync3n_signs_times_v",
+    "clm_clmatrix_subset_device", "clm_clmatrix_subset", "clm_partner_subset", "clm_assemble_subset",
+    "clm_shift_equations_csc_subset",
 )
 
 # void (*)(void *user, int level, const char *msg)
@@ -266,6 +268,20 @@ def load():
     L.clm_set_log.argtypes = [vp, LOG_FN, vp]
     L.clm_clstack_mask.restype = ctypes.c_int
     L.clm_clstack_mask.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.clm_clmatrix_subset_device.restype = ctypes.c_int
+    L.clm_clmatrix_subset_device.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp, vp,
+                                             ctypes.POINTER(ClmStats)]
+    L.clm_clmatrix_subset.restype = ctypes.c_int
+    L.clm_clmatrix_subset.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp, vp,
+                                      ctypes.POINTER(ClmStats)]
+    L.clm_partner_subset.restype = ctypes.c_int
+    L.clm_partner_subset.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, vp,
+                                     ctypes.POINTER(ctypes.c_int64)]
+    L.clm_assemble_subset.restype = ctypes.c_int
+    L.clm_assemble_subset.argtypes = [ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp]
+    L.clm_shift_equations_csc_subset.restype = ctypes.c_int
+    L.clm_shift_equations_csc_subset.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp,
+                                                 ctypes.POINTER(ctypes.c_int64)]
     L.clm_pft_freqs.restype = ctypes.c_int
     L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L

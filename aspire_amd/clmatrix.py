@@ -61,7 +61,7 @@ def _as_matlab_pf(pf: np.ndarray):
 
 
 def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="fp16", device=0,
-                     tie_margin=1e-3, variant=0, verbose=0, flags=0, map_filter_radius=0):
+                     tie_margin=1e-3, variant=0, verbose=0, flags=0, map_filter_radius=0, sel=None):
     """Per-pair records of the common-lines search.
 
     ``pf`` is a numpy array (host path) or a torch tensor on a HIP device of
@@ -73,7 +73,13 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="fp16", device=
     input, device tensors for tensor input.  ``map_filter_radius`` d > 0
     filters every map with cryo_average_clmap before the search (FP64 path,
     ``clm_clmatrix_filtered``); ``corr`` is then the filtered value.
+    ``sel`` (P bytes in equation order, nonzero = search the pair; a device
+    tensor for device input) restricts the search to those pairs
+    (``clm_clmatrix_subset``): the others come back with cl1 = cl2 = 0,
+    j = -1, corr = shift = 0, and ``stats["pairs"]`` counts the selected.
     """
+    if sel is not None and map_filter_radius:
+        raise ValueError("the partner-subset search has no filtered-map form")
     ctx = _lib.context(device)
     stats = _lib.ClmStats()
     if type(pf).__module__.startswith("torch"):
@@ -95,7 +101,13 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="fp16", device=
                 ctypes.c_void_p(out["cl1"].data_ptr()), ctypes.c_void_p(out["cl2"].data_ptr()),
                 ctypes.c_void_p(out["j"].data_ptr()), ctypes.c_void_p(out["corr"].data_ptr()),
                 ctypes.c_void_p(out["shift"].data_ptr()), ctypes.byref(stats))
-        if map_filter_radius:
+        if sel is not None:
+            if not (sel.is_cuda and sel.dtype == torch.uint8 and sel.is_contiguous() and sel.numel() == P
+                    and sel.device == dev):
+                raise ValueError(f"sel must be a contiguous uint8 tensor of {P} bytes on {dev}")
+            rc = _lib.load().clm_clmatrix_subset_device(ctx.handle, ctypes.byref(p), args[0],
+                                                        ctypes.c_void_p(sel.data_ptr()), *args[1:])
+        elif map_filter_radius:
             rc = _lib.load().clm_clmatrix_filtered_device(ctx.handle, ctypes.byref(p), int(map_filter_radius),
                                                           *args)
         else:
@@ -113,7 +125,13 @@ def clmatrix_records(pf, max_shift=15, shift_step=1.0, precision="fp16", device=
             out["cl1"].ctypes.data_as(ctypes.c_void_p), out["cl2"].ctypes.data_as(ctypes.c_void_p),
             out["j"].ctypes.data_as(ctypes.c_void_p), out["corr"].ctypes.data_as(ctypes.c_void_p),
             out["shift"].ctypes.data_as(ctypes.c_void_p), ctypes.byref(stats))
-    if map_filter_radius:
+    if sel is not None:
+        selh = np.ascontiguousarray(sel, dtype=np.uint8)
+        if selh.shape != (P,):
+            raise ValueError(f"sel must hold {P} bytes")
+        rc = _lib.load().clm_clmatrix_subset(ctx.handle, ctypes.byref(p), args[0],
+                                             selh.ctypes.data_as(ctypes.c_void_p), *args[1:])
+    elif map_filter_radius:
         rc = _lib.load().clm_clmatrix_filtered(ctx.handle, ctypes.byref(p), int(map_filter_radius), *args)
     else:
         rc = _lib.load().clm_clmatrix(ctx.handle, ctypes.byref(p), *args)
@@ -208,12 +226,18 @@ def autotune(pf, max_shift=15, shift_step=1.0, precision="fp16", device=0, sampl
     return rows, best
 
 
-def assemble(K: int, n_theta: int, rec):
+def assemble(K: int, n_theta: int, rec, sel=None):
     """Reference outputs from per-pair records (FP64, host; libclm's
-    clm_assemble and clm_shift_equations_csc)."""
+    clm_assemble and clm_shift_equations_csc).  With ``sel`` (P bytes), the
+    partner-subset form: equations for the selected pairs only, numbered in
+    pair order (clm_assemble_subset, clm_shift_equations_csc_subset)."""
     import scipy.sparse as sp
 
     L = _lib.load()
+    if sel is not None:
+        selh = np.ascontiguousarray(sel, dtype=np.uint8)
+        if selh.shape != (max(K * (K - 1) // 2, 0),):
+            raise ValueError("sel must hold K(K-1)/2 bytes")
     cl1 = np.ascontiguousarray(rec["cl1"], dtype=np.int32)
     cl2 = np.ascontiguousarray(rec["cl2"], dtype=np.int32)
     corr = np.ascontiguousarray(rec["corr"], dtype=np.float64)
@@ -222,23 +246,28 @@ def assemble(K: int, n_theta: int, rec):
     corrstack = np.zeros((K, K), order="F")
     emap = np.zeros((K, K), order="F")
     vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    rc = L.clm_assemble(int(K), vp(cl1), vp(cl2), vp(corr), vp(clstack), vp(corrstack), vp(emap))
+    if sel is None:
+        rc = L.clm_assemble(int(K), vp(cl1), vp(cl2), vp(corr), vp(clstack), vp(corrstack), vp(emap))
+        nrow = K * (K - 1) // 2
+        csc = lambda *a: L.clm_shift_equations_csc(int(K), int(n_theta), *a)
+    else:
+        rc = L.clm_assemble_subset(int(K), vp(selh), vp(cl1), vp(cl2), vp(corr), vp(clstack), vp(corrstack),
+                                   vp(emap))
+        nrow = int(np.count_nonzero(selh))
+        csc = lambda *a: L.clm_shift_equations_csc_subset(int(K), int(n_theta), vp(selh), *a)
     if rc:
         raise _lib.ClmError(f"clm_assemble failed ({rc})")
-    P = K * (K - 1) // 2
     nnz = ctypes.c_int64(0)
     jc = np.zeros(2 * K + 2, np.int64)
-    rc = L.clm_shift_equations_csc(int(K), int(n_theta), vp(cl1), vp(cl2), vp(shift), vp(jc),
-                                   None, None, ctypes.byref(nnz))
+    rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), None, None, ctypes.byref(nnz))
     if rc:
         raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
     ir = np.zeros(nnz.value, np.int64)
     pr = np.zeros(nnz.value, np.float64)
-    rc = L.clm_shift_equations_csc(int(K), int(n_theta), vp(cl1), vp(cl2), vp(shift), vp(jc),
-                                   vp(ir), vp(pr), ctypes.byref(nnz))
+    rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), vp(ir), vp(pr), ctypes.byref(nnz))
     if rc:
         raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
-    eqs = sp.csc_matrix((pr, ir, jc), shape=(P, 2 * K + 1))
+    eqs = sp.csc_matrix((pr, ir, jc), shape=(nrow, 2 * K + 1))
     return clstack, corrstack, eqs, emap
 
 
@@ -294,6 +323,53 @@ def cryo_clmatrix(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, map_filter
             P = K * (K - 1) // 2
             log.info("Matched common-lines=%d/%d  (%3.1f%%)", matched, P, 100.0 * matched / max(P, 1))
     return clstack, corrstack, eqs, emap, mask
+
+
+def partner_subset(K: int, NK: int, seed: int = 0):
+    """(sel, count): the pairs cryo_clmatrix_gpu_1 searches
+    (sinograms/cryo_clmatrix_gpu_1.m:150-151,346-350) -- image k1 with
+    min(K-1-k1, NK) of its later images drawn uniformly without replacement;
+    NK = -1 selects every pair.  libclm's clm_partner_subset (host); the draw
+    depends on (seed, k1) only."""
+    K, NK = int(K), int(NK)
+    P = max(K * (K - 1) // 2, 0)
+    sel = np.zeros(P, np.uint8)
+    n = ctypes.c_int64(0)
+    rc = _lib.load().clm_partner_subset(K, NK, ctypes.c_uint64(int(seed) & (2 ** 64 - 1)),
+                                        sel.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n))
+    if rc:
+        raise _lib.ClmError(f"clm_partner_subset failed ({rc}): K={K}, NK={NK}")
+    return sel, int(n.value)
+
+
+def cryo_clmatrix_subset(pf, NK=-1, verbose=1, max_shift=15, shift_step=1.0, *, seed=0, sel=None,
+                         precision="fp16", device=0):
+    """The partner-subset form of ``cryo_clmatrix_gpu_1(pf, NK, verbose,
+    max_shift, shift_step)`` (sinograms/cryo_clmatrix_gpu_1.m:1-24): each image
+    is searched against at most ``NK`` of its later images, drawn at random
+    (``partner_subset``; or the pairs of ``sel``, P bytes in equation order).
+    Each selected pair takes the drop-in's search (cryo_clmatrix_cpu.m:360-399
+    semantics, the same parity); the other pairs keep clstack = corrstack = 0.
+    shift_equations holds one row per selected pair in pair order
+    (:809-827,853-858) and shift_equations_map its 1-based number.  Returns
+    the reference's five outputs (clstack_mask zeros) and the selection."""
+    if hasattr(pf, "shape") and len(pf.shape) >= 2 and pf.shape[1] % 2:
+        raise ValueError("n_theta must be even")
+    shp = tuple(pf.shape)
+    n_theta, K = shp[1], (shp[2] if len(shp) == 3 else 1)
+    if sel is None:
+        sel, _ = partner_subset(K, NK, seed)
+    sel = np.ascontiguousarray(sel, dtype=np.uint8)
+    if type(pf).__module__.startswith("torch"):
+        import torch
+
+        dsel = torch.as_tensor(sel).to(pf.device)
+        rec, stats = clmatrix_records(pf, max_shift, shift_step, precision, device, verbose=verbose, sel=dsel)
+        rec = {k: v.cpu().numpy() for k, v in rec.items()}
+    else:
+        rec, stats = clmatrix_records(pf, max_shift, shift_step, precision, device, verbose=verbose, sel=sel)
+    clstack, corrstack, eqs, emap = assemble(K, n_theta, rec, sel=sel)
+    return clstack, corrstack, eqs, emap, np.zeros((K, K), order="F"), sel
 
 
 def clstack_mask(clstack, ref_clmatrix, n_theta):

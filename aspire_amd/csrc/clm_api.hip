@@ -1276,6 +1276,64 @@ int run_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, const float*
     return CLM_OK;
 }
 
+// Partner-subset search: the pairs flagged in d_sel (one byte per pair, in
+// equation order), each with the full search of the other entry points; the
+// rest keep the outputs of a pair with no common line.  The flags become
+// explicit pass-1 tiles on the device, as the windowed search's fallback
+// pairs do.
+int run_subset(clm_ctx* c, const clm_params* p, const uint8_t* d_sel, const clm_stack_io& out, clm_stats* st) {
+    int rc;
+    if (st) std::memset(st, 0, sizeof *st);
+    StackJob probe;
+    if ((rc = plan_stack(c, p, probe))) return rc;
+    const long long K = p->K, P = probe.P;
+    if (P == 0) return CLM_OK;
+    if (!out.pf || !d_sel || !out.cl1 || !out.cl2 || !out.jidx || !out.corr || !out.shift1d)
+        return fail(c, CLM_E_ARG, "null pf, selection or output");
+    const int ppw = probe.v.W / probe.v.CPP;
+    if ((rc = ensure(c, c->w_rows, (size_t)(4 * K + 4) * 4)) ||
+        (rc = ensure(c, c->w_list, (size_t)P * sizeof(SlotRef))) ||
+        (rc = ensure(c, c->w_plist, (size_t)P * sizeof(int4))) ||
+        (rc = ensure(c, c->w_tiles, (size_t)P * sizeof(int4))))
+        return rc;
+    int* rows = (int*)c->w_rows.p;
+    int *cnt = rows, *tcnt = rows + K, *off = rows + 2 * K, *toff = rows + 3 * K, *tot = rows + 4 * K;
+    const unsigned ig = (unsigned)std::min<long long>((P + 255) / 256, (long long)c->n_cu * 16);
+    hipLaunchKernelGGL(subset_init_kernel, dim3(ig), dim3(256), 0, c->stream, P, d_sel, out.cl1, out.cl2, out.jidx,
+                       out.corr, out.shift1d);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(row_count_kernel, dim3((unsigned)(K - 1)), dim3(256), 0, c->stream, (int)K, d_sel, ppw, cnt,
+                       tcnt);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(row_scan_kernel, dim3(1), dim3(1024), 0, c->stream, (int)(K - 1), (const int*)cnt,
+                       (const int*)tcnt, off, toff, tot);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(row_emit_kernel, dim3((unsigned)(K - 1)), dim3(256), 0, c->stream, (int)K, 0, d_sel, ppw,
+                       (const int*)off, (const int*)toff, (SlotRef*)c->w_list.p, (int4*)c->w_plist.p,
+                       (int4*)c->w_tiles.p);
+    HIPCHK(c, hipGetLastError());
+    int nsel = 0;
+    HIPCHK(c, hipMemcpyAsync(&nsel, tot, sizeof nsel, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    StackJob xs;
+    xs.sel_list = (const SlotRef*)c->w_list.p;
+    xs.sel_count = tot;
+    xs.sel_tiles = (const int4*)c->w_tiles.p;
+    xs.sel_plist = (const int4*)c->w_plist.p;
+    xs.sel_ntiles = tot + 1;
+    Plan pl;
+    pl.twin_lazy = !c->arena;
+    if ((rc = prepare(c, pl, 1, p, &out, true, &xs, 0))) return rc;
+    if (!pl.jobs[0].exact && pl.jobs[0].v.W / pl.jobs[0].v.CPP != ppw)
+        return fail(c, CLM_E_HIP, "subset tile shape changed");
+    begin_stats(pl, st);
+    if (st) st->pairs = nsel;
+    if (nsel == 0) return CLM_OK;
+    if ((rc = run_plan(c, pl, true, st))) return rc;
+    timed_stats(c, st);
+    return CLM_OK;
+}
+
 // Candidate variants of a shape: every (CT, W) the kernels support, W a
 // multiple of the waves per pair between 4 and 8.
 std::vector<std::pair<int, int>> candidate_variants(int opk, int NCT, int Rp) {
@@ -1461,6 +1519,62 @@ void host_parallel(long long n, long long grain, F f) {
     for (int t = started + 1; t < nt; t++) f(n * t / nt, n * (t + 1) / nt);
     for (auto& x : th) x.join();
 }
+
+// SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): the per-row generator of
+// clm_partner_subset, so that each row's draw depends on (seed, row) only.
+inline uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Equation number of every pair among the selected ones (0-based; the
+// unselected pairs' entries are not used).  Null without a selection.
+std::vector<int32_t> selected_rank(long long P, const uint8_t* sel) {
+    std::vector<int32_t> r;
+    if (!sel) return r;
+    r.resize((size_t)P);
+    int32_t n = 0;
+    for (long long e = 0; e < P; e++) {
+        r[(size_t)e] = n;
+        n += sel[e] != 0;
+    }
+    return r;
+}
+
+int assemble_impl(int32_t K, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2, const double* corr,
+                  double* clstack, double* corrstack, double* eqmap) {
+    if (K < 0) return CLM_E_ARG;
+    const std::vector<int32_t> rank = selected_rank(clm_num_pairs(K), eqmap ? sel : nullptr);
+    // one column of the K x K column-major outputs per step, columns split
+    // over host threads: entry (r, c) is pair (r, c) above the diagonal
+    // (cl1, 1 - corr, its equation number), pair (c, r) below it (cl2, 0, 0)
+    host_parallel(K, 65536 / std::max(K, 1) + 1, [&](long long c0, long long c1) {
+        for (long long c = c0; c < c1; c++) {
+            const size_t col = (size_t)c * K;
+            for (long long r = 0; r < K; r++) {
+                double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+                if (r < c) {
+                    const long long e = pair_slot(r, c, K);
+                    v1 = cl1 ? cl1[e] : 0;
+                    v2 = corr && corr[e] != 0.0 ? 1.0 - corr[e] : 0.0;  // :764
+                    if (!sel) v3 = (double)(e + 1);
+                    else if (sel[e]) v3 = (double)rank[(size_t)e] + 1.0;
+                } else if (r > c) {
+                    v1 = cl2 ? cl2[pair_slot(c, r, K)] : 0;
+                }
+                if (clstack) clstack[col + r] = v1;
+                if (corrstack) corrstack[col + r] = v2;
+                if (eqmap) eqmap[col + r] = v3;
+            }
+        }
+    });
+    return CLM_OK;
+}
+
+int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2,
+                         const double* shift1d, int64_t* jc, int64_t* ir, double* pr, int64_t* nnz);
 }  // namespace
 
 extern "C" {
@@ -1589,6 +1703,20 @@ int clm_clmatrix_windowed(clm_ctx* c, const clm_params* p, const void* d_pf, con
     return run_windowed(c, p, d_pf, d_prop, window, gate, out, d_flag, ws);
 }
 
+int clm_clmatrix_subset_device(clm_ctx* c, const clm_params* p, const void* d_pf, const uint8_t* d_sel,
+                               int32_t* d_cl1, int32_t* d_cl2, int32_t* d_jidx, double* d_corr, double* d_shift1d,
+                               clm_stats* st) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    HIPCHK(c, hipSetDevice(c->device));
+    const clm_stack_io out{d_pf, d_cl1, d_cl2, d_jidx, d_corr, d_shift1d};
+    clm_stats local;
+    if (!st && c->log_fn) st = &local;
+    const int rc = run_subset(c, p, d_sel, out, st);
+    if (rc == CLM_OK && st && st->pairs > 0) log_job(c, 1, p, st);
+    return rc;
+}
+
 int clm_autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t sample_pairs, clm_tune_entry* entries,
                  int32_t max_entries, int32_t* n_entries) {
     if (!c) return CLM_E_ARG;
@@ -1644,6 +1772,41 @@ static int host_job(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl
 int clm_clmatrix(clm_ctx* c, const clm_params* p, const void* pf, int32_t* cl1, int32_t* cl2, int32_t* jidx,
                  double* corr, double* shift1d, clm_stats* st) {
     return host_job(c, p, pf, cl1, cl2, jidx, corr, shift1d, st, nullptr);
+}
+
+int clm_clmatrix_subset(clm_ctx* c, const clm_params* p, const void* pf, const uint8_t* sel, int32_t* cl1,
+                        int32_t* cl2, int32_t* jidx, double* corr, double* shift1d, clm_stats* st) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    if (p->n_theta % 2) return fail(c, CLM_E_ARG, "n_theta must be even");
+    if (p->K < 0 || p->n_r < 0 || p->n_theta < 0) return fail(c, CLM_E_ARG, "negative size");
+    const long long P = clm_num_pairs(p->K);
+    if (P > 0 && (!pf || !sel || !cl1 || !cl2 || !jidx || !corr || !shift1d))
+        return fail(c, CLM_E_ARG, "null pf, selection or output");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t esz = p->pf_dtype == CLM_C64 ? 8 : 16;
+    const size_t pf_bytes = (size_t)p->n_r * p->n_theta * p->K * esz;
+    int rc;
+    if ((rc = ensure(c, c->pf, pf_bytes)) || (rc = ensure(c, c->w_flag, (size_t)P)) ||
+        (rc = ensure(c, c->out_cl1, P * 4)) || (rc = ensure(c, c->out_cl2, P * 4)) ||
+        (rc = ensure(c, c->out_j, P * 4)) || (rc = ensure(c, c->out_corr, P * 8)) ||
+        (rc = ensure(c, c->out_shift, P * 8)))
+        return rc;
+    if (pf_bytes) HIPCHK(c, hipMemcpyAsync(c->pf.p, pf, pf_bytes, hipMemcpyHostToDevice, c->stream));
+    if (P > 0) HIPCHK(c, hipMemcpyAsync(c->w_flag.p, sel, (size_t)P, hipMemcpyHostToDevice, c->stream));
+    rc = clm_clmatrix_subset_device(c, p, c->pf.p, (const uint8_t*)c->w_flag.p, (int32_t*)c->out_cl1.p,
+                                    (int32_t*)c->out_cl2.p, (int32_t*)c->out_j.p, (double*)c->out_corr.p,
+                                    (double*)c->out_shift.p, st);
+    if (rc) return rc;
+    if (P > 0) {
+        HIPCHK(c, hipMemcpyAsync(cl1, c->out_cl1.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(cl2, c->out_cl2.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(jidx, c->out_j.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(corr, c->out_corr.p, P * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(shift1d, c->out_shift.p, P * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return CLM_OK;
 }
 
 static int check_radius(clm_ctx* c, int32_t r) {
@@ -1717,34 +1880,74 @@ int clm_clstack_mask(int32_t K, int32_t n_theta, const double* clstack, const do
 
 int clm_assemble(int32_t K, const int32_t* cl1, const int32_t* cl2, const double* corr, double* clstack,
                  double* corrstack, double* eqmap) {
-    if (K < 0) return CLM_E_ARG;
-    // one column of the K x K column-major outputs per step, columns split
-    // over host threads: entry (r, c) is pair (r, c) above the diagonal
-    // (cl1, 1 - corr, its row number), pair (c, r) below it (cl2, 0, 0)
-    host_parallel(K, 65536 / std::max(K, 1) + 1, [&](long long c0, long long c1) {
-        for (long long c = c0; c < c1; c++) {
-            const size_t col = (size_t)c * K;
-            for (long long r = 0; r < K; r++) {
-                double v1 = 0.0, v2 = 0.0, v3 = 0.0;
-                if (r < c) {
-                    const long long e = pair_slot(r, c, K);
-                    v1 = cl1 ? cl1[e] : 0;
-                    v2 = corr && corr[e] != 0.0 ? 1.0 - corr[e] : 0.0;  // :764
-                    v3 = (double)(e + 1);
-                } else if (r > c) {
-                    v1 = cl2 ? cl2[pair_slot(c, r, K)] : 0;
-                }
-                if (clstack) clstack[col + r] = v1;
-                if (corrstack) corrstack[col + r] = v2;
-                if (eqmap) eqmap[col + r] = v3;
+    return assemble_impl(K, nullptr, cl1, cl2, corr, clstack, corrstack, eqmap);
+}
+
+int clm_assemble_subset(int32_t K, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2, const double* corr,
+                        double* clstack, double* corrstack, double* eqmap) {
+    if (K > 1 && !sel) return CLM_E_ARG;
+    return assemble_impl(K, sel, cl1, cl2, corr, clstack, corrstack, eqmap);
+}
+
+int clm_partner_subset(int32_t K, int32_t NK, uint64_t seed, uint8_t* sel, int64_t* count) {
+    if (K < 0 || NK < -1) return CLM_E_ARG;
+    const long long P = clm_num_pairs(K);
+    if (P > 0 && !sel) return CLM_E_ARG;
+    // cryo_clmatrix_gpu_1.m:150-151,346-350: image k1 (0-based) takes
+    // n2 = min(K-1-k1, NK) of its K-1-k1 later images, drawn uniformly
+    // without replacement and kept in increasing order; NK = -1 means all.
+    // Selection sampling (Knuth, TAOCP vol. 2, 3.4.2, Algorithm S) visits the
+    // candidates in order and takes each with probability (needed / left),
+    // which yields every n2-subset with equal probability, already sorted.
+    std::vector<long long> taken(K > 1 ? K - 1 : 0, 0);
+    host_parallel(K > 1 ? K - 1 : 0, 64, [&](long long r0, long long r1) {
+        for (long long k1 = r0; k1 < r1; k1++) {
+            const long long m = K - 1 - k1;
+            const long long n2 = NK < 0 ? m : std::min<long long>(m, NK);
+            uint8_t* row = sel + pair_slot(k1, k1 + 1, K);
+            if (n2 == m) {
+                std::memset(row, 1, (size_t)m);
+                taken[k1] = m;
+                continue;
             }
+            uint64_t st = seed ^ (0xd1b54a32d192ed03ull * (uint64_t)(k1 + 1));
+            long long got = 0;
+            for (long long i = 0; i < m; i++) {
+                const double u = (double)(splitmix64(st) >> 11) * 0x1p-53;  // [0, 1)
+                const bool take = (double)(m - i) * u < (double)(n2 - got);
+                row[i] = take ? 1 : 0;
+                got += take;
+            }
+            taken[k1] = got;
         }
     });
+    if (count) {
+        long long n = 0;
+        for (long long t : taken) n += t;
+        *count = n;
+    }
     return CLM_OK;
 }
 
 int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, const int32_t* cl2,
                             const double* shift1d, int64_t* jc, int64_t* ir, double* pr, int64_t* nnz) {
+    return shift_equations_impl(K, n_theta, nullptr, cl1, cl2, shift1d, jc, ir, pr, nnz);
+}
+
+int clm_shift_equations_csc_subset(int32_t K, int32_t n_theta, const uint8_t* sel, const int32_t* cl1,
+                                   const int32_t* cl2, const double* shift1d, int64_t* jc, int64_t* ir, double* pr,
+                                   int64_t* nnz) {
+    if (K > 1 && !sel) return CLM_E_ARG;
+    return shift_equations_impl(K, n_theta, sel, cl1, cl2, shift1d, jc, ir, pr, nnz);
+}
+
+}  // extern "C"
+
+namespace {
+// shift_equations in CSC form; with a selection, only the selected pairs'
+// rows, numbered in pair order (cryo_clmatrix_gpu_1.m:809-827,853-858)
+int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2,
+                         const double* shift1d, int64_t* jc, int64_t* ir, double* pr, int64_t* nnz) {
     if (K < 0 || n_theta <= 0 || n_theta % 2 || !cl1 || !cl2 || !shift1d) return CLM_E_ARG;
     const int L = n_theta / 2;
     const double dtheta = M_PI / (double)L;  // :245
@@ -1772,6 +1975,8 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
     };
     const long long P = clm_num_pairs(K);
     const long long ncol = 2 * (long long)K;
+    const std::vector<int32_t> rank = selected_rank(P, sel);
+    auto row_of = [&](long long e) -> long long { return sel ? (long long)rank[(size_t)e] : e; };
     // column q = 2k + xy (image k, coordinate xy): rows where k is the second
     // image (pairs (j, k), j < k), then rows where it is the first (pairs
     // (k, m), m > k), nonzero coefficients only (MATLAB's sparse keeps no
@@ -1784,19 +1989,21 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
         double c4[4];
         for (long long j = 0; j < k; j++) {
             const long long e = pair_slot(j, k, K);
+            if (sel && !sel[e]) continue;
             coefs(e, c4);
             const double v = c4[2 + xy];
             if (v != 0.0) {
-                if (out_ir) { out_ir[n] = e; out_pr[n] = v; }
+                if (out_ir) { out_ir[n] = row_of(e); out_pr[n] = v; }
                 n++;
             }
         }
         for (long long m = k + 1; m < K; m++) {
             const long long e = pair_slot(k, m, K);
+            if (sel && !sel[e]) continue;
             coefs(e, c4);
             const double v = c4[xy];
             if (v != 0.0) {
-                if (out_ir) { out_ir[n] = e; out_pr[n] = v; }
+                if (out_ir) { out_ir[n] = row_of(e); out_pr[n] = v; }
                 n++;
             }
         }
@@ -1812,7 +2019,7 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
     host_parallel(nblk, 1, [&](long long b0, long long b1) {
         for (long long b = b0; b < b1; b++) {
             long long n = 0;
-            for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++) n += shift1d[e] != 0.0;
+            for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++) n += shift1d[e] != 0.0 && (!sel || sel[e]);
             rcnt[b] = n;
         }
     });
@@ -1834,8 +2041,8 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
             for (long long b = b0; b < b1; b++) {
                 long long n = rstart[b];
                 for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++)
-                    if (shift1d[e] != 0.0) {
-                        ir[n] = e;
+                    if (shift1d[e] != 0.0 && (!sel || sel[e])) {
+                        ir[n] = row_of(e);
                         pr[n] = shift1d[e];
                         n++;
                     }
@@ -1844,5 +2051,4 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
     }
     return CLM_OK;
 }
-
-}  // extern "C"
+}  // namespace

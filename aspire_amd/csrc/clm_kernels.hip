@@ -668,6 +668,24 @@ __global__ void window_merge_kernel(int K, int L, const double* __restrict__ lsh
     }
 }
 
+// Partner-subset search: the pairs left out keep the outputs of a pair with
+// no common line (cryo_clmatrix_gpu_1.m:346-350 searches only the drawn
+// partners; clstack and corrstack stay zero elsewhere).  The selected pairs'
+// outputs are written by the search itself.
+__global__ void subset_init_kernel(long long P, const uint8_t* __restrict__ sel, int32_t* __restrict__ cl1,
+                                   int32_t* __restrict__ cl2, int32_t* __restrict__ jidx, double* __restrict__ corr,
+                                   double* __restrict__ shift1d) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < P; e += stride) {
+        if (sel[e]) continue;
+        cl1[e] = 0;
+        cl2[e] = 0;
+        jidx[e] = -1;
+        corr[e] = 0.0;
+        shift1d[e] = 0.0;
+    }
+}
+
 // Flagged pairs -> explicit pass-1 tiles, in pair order.  Three steps: per
 // row counts, one scan, per row ordered compaction.
 __global__ void row_count_kernel(int K, const uint8_t* __restrict__ flag, int ppw, int* __restrict__ cnt,

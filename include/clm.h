@@ -274,6 +274,41 @@ int clm_clmatrix_windowed(clm_ctx *ctx, const clm_params *p, const void *d_pf,
                           clm_window_stats *stats);
 
 /*
+ * Partner subset: the search of chosen pairs only, as
+ * cryo_clmatrix_gpu_1(pf, NK, ...) (sinograms/cryo_clmatrix_gpu_1.m:22-24,
+ * 346-350) searches each image against at most NK of its later images.
+ * sel holds one byte per pair in equation order ((k1,k2), k1 < k2, k1
+ * outer); a nonzero byte selects the pair (clm_partner_subset draws the
+ * reference's subset).  Each selected pair is searched exactly as by
+ * clm_clmatrix_device (cryo_clmatrix_cpu.m:360-399 semantics and parity);
+ * every other pair gets cl1 = cl2 = 0, jidx = -1, corr = shift1d = 0 (the
+ * reference leaves clstack and corrstack at zero there).  stats->pairs is
+ * the number of selected pairs.  The _device form takes device pointers
+ * (pf, sel and outputs); the other, host buffers.
+ */
+int clm_clmatrix_subset_device(clm_ctx *ctx, const clm_params *p,
+                               const void *d_pf, const uint8_t *d_sel,
+                               int32_t *d_cl1, int32_t *d_cl2, int32_t *d_jidx,
+                               double *d_corr, double *d_shift1d,
+                               clm_stats *stats);
+int clm_clmatrix_subset(clm_ctx *ctx, const clm_params *p, const void *pf,
+                        const uint8_t *sel, int32_t *cl1, int32_t *cl2,
+                        int32_t *jidx, double *corr, double *shift1d,
+                        clm_stats *stats);
+
+/*
+ * The partner subset of cryo_clmatrix_gpu_1.m:150-151,346-350 (host):
+ * image k1 (0-based) is paired with n2 = min(K-1-k1, NK) of its K-1-k1
+ * later images, drawn uniformly without replacement (every n2-subset equally
+ * likely) and kept in increasing order; NK = -1 selects every pair, NK = 0
+ * none.  The draw depends on (seed, k1) only: MATLAB's randperm stream is
+ * not reproduced, its distribution is.  sel receives P bytes (1 = selected);
+ * count (optional) the number selected.
+ */
+int clm_partner_subset(int32_t K, int32_t NK, uint64_t seed, uint8_t *sel,
+                       int64_t *count);
+
+/*
  * Shift network of the learned form (no counterpart in the reference).  It
  * reads the CLM_SN_NR lowest radial samples of all CLM_SN_T rays of each
  * image and proposes the image's 2-D shift for clm_clmatrix_windowed.
@@ -345,6 +380,23 @@ int clm_assemble(int32_t K, const int32_t *cl1, const int32_t *cl2,
 int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t *cl1,
                             const int32_t *cl2, const double *shift1d,
                             int64_t *jc, int64_t *ir, double *pr, int64_t *nnz);
+
+/*
+ * Outputs of a partner-subset search (cryo_clmatrix_gpu_1.m:809-827,
+ * 853-858): as clm_assemble and clm_shift_equations_csc, with equations for
+ * the selected pairs only, numbered in pair order.  shift_equations is
+ * n_sel x (2K+1) (n_sel = number of nonzero bytes of sel);
+ * shift_equations_map(k1,k2) is the 1-based equation number of a selected
+ * pair and 0 elsewhere.
+ */
+int clm_assemble_subset(int32_t K, const uint8_t *sel, const int32_t *cl1,
+                        const int32_t *cl2, const double *corr,
+                        double *clstack, double *corrstack, double *eqmap);
+int clm_shift_equations_csc_subset(int32_t K, int32_t n_theta,
+                                   const uint8_t *sel, const int32_t *cl1,
+                                   const int32_t *cl2, const double *shift1d,
+                                   int64_t *jc, int64_t *ir, double *pr,
+                                   int64_t *nnz);
 
 /*
  * Polar Fourier transform of a projection stack on the device: the step
