@@ -129,6 +129,15 @@ __device__ inline double dpp_d(double x) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// dpp_d without the old value: for the in-row patterns below every lane has
+// a source lane, so nothing of the old value survives
+template <int CTRL>
+__device__ inline double mov_dpp_d(double x) {
+    const unsigned long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffu), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 __device__ inline double readlane_d(double x, int l) {
     const unsigned long long u = __double_as_longlong(x);
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffu), l);
@@ -147,27 +156,75 @@ __device__ inline double wave_sum(double x) {
     x += dpp_d<0x128>(x);
     return ((readlane_d(x, 0) + readlane_d(x, 16)) + readlane_d(x, 32)) + readlane_d(x, 48);
 }
+// wave_sum of two values at once (the same order and bits as two calls),
+// the steps interleaved so each fills the other's DPP wait
+__device__ inline void wave_sum2(double& x, double& y) {
+    x += mov_dpp_d<0xb1>(x);
+    y += mov_dpp_d<0xb1>(y);
+    x += mov_dpp_d<0x4e>(x);
+    y += mov_dpp_d<0x4e>(y);
+    x += mov_dpp_d<0x124>(x);
+    y += mov_dpp_d<0x124>(y);
+    x += mov_dpp_d<0x128>(x);
+    y += mov_dpp_d<0x128>(y);
+    x = ((readlane_d(x, 0) + readlane_d(x, 16)) + readlane_d(x, 32)) + readlane_d(x, 48);
+    y = ((readlane_d(y, 0) + readlane_d(y, 16)) + readlane_d(y, 32)) + readlane_d(y, 48);
+}
+// Sum of p[0..n) in one fixed order by one whole wave: lane l adds p[l],
+// p[l + 64], ... in order, then wave_sum.  Every caller gets the same bits.
+__device__ inline double wave_fixed_sum(const double* p, int n) {
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    int i = lane;
+    // eight loads in flight, then their adds in order
+    for (; i + 7 * 64 < n; i += 8 * 64) {
+        double a[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) a[q] = p[i + q * 64];
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += a[q];
+    }
+    for (; i < n; i += 64) s += p[i];
+    return wave_sum(s);
+}
 
-// One pass over tile t's pairs (row-major, pair e = first(k1) + k2 - k1 - 1):
-//   INIT: pack[e] = cl1 | cl2 << 16 (both 2L+1 when either is outside
-//         [0, 2L]) and u[e] = b[e];
-//   else: u[e] <- (A v)[e] - (alpha/beta) u[e]  (u holds the previous
+// One pass over tile t's pairs: pair (k1, k2) at row r = k1 - TR*kb and
+// column c = k2 - TC*cb of the tile is equation e = first(k1) + k2 - k1 - 1
+// and slot q = (t*TR + r)*TC + c of the tile-major pack and u:
+//   INIT: pack[q] = cl1[e] | cl2[e] << 16 (both 2L+1 when either is outside
+//         [0, 2L]) and u[q] = b[e];
+//   else: u[q] <- (A v)[e] - (alpha/beta) u[q]  (u holds the previous
 //         unnormalized vector, whose norm is beta);
 // and, of the new u: normp[t] = sum of squares; rowp[cb][2k1 + c] = the
 // tile's row part of A'u (the k1 coefficients), colp[kb][2k2 + c] = its
 // column part (the k2 coefficients).  A thread owns one column and walks the
 // tile's rows in order; row sums go through fixed wave butterflies and then
 // the four waves in order.
-template <bool INIT>
+// LT: the two coefficient tables (5 (2L+2) doubles, ta then tb, contiguous)
+// are first copied to LDS, so the loop's table reads wait on the LDS counter
+// and not behind the next rows' HBM loads on the vector-memory counter.
+template <bool INIT, bool LT>
 __global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __restrict__ tiles,
                                                    const int* __restrict__ cl1, const int* __restrict__ cl2,
                                                    const double* __restrict__ b, unsigned* __restrict__ pack,
                                                    const double2* __restrict__ ta, const double* __restrict__ tb,
-                                                   const double* __restrict__ v, const State* S, double* u,
+                                                   const double* __restrict__ v, State* S, double* u,
                                                    double* normp, double* rowp, double* colp) {
     __shared__ double s_row[TR][4][2];
     __shared__ double s_sq[4];
+    extern __shared__ double s_tab[];
     if (!INIT && S->done) return;
+    // the tile's rows of v (2 TR entries, zero past the last image)
+    __shared__ double s_v[2 * TR];
+    if (!INIT && threadIdx.x < 2 * TR) {
+        const int i = 2 * TR * tiles[blockIdx.x].x + threadIdx.x;
+        s_v[threadIdx.x] = i < 2 * K ? v[i] : 0.0;
+    }
+    if constexpr (LT) {
+        const double* src = (const double*)ta;
+        for (int i = threadIdx.x; i < 5 * (2 * L + 2); i += blockDim.x) s_tab[i] = src[i];
+    }
+    if (LT || !INIT) __syncthreads();
     const int2 tl = tiles[blockIdx.x];
     const int kb = tl.x, cb = tl.y;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -181,44 +238,93 @@ __global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __r
     }
     double c0 = 0.0, c1 = 0.0, sq = 0.0;
     const unsigned z = (unsigned)(2 * L + 1);
-#pragma unroll 8
-    for (int r = 0; r < TR; r++) {
-        const int k1 = TR * kb + r;
-        double r0 = 0.0, r1 = 0.0;
-        if (k1 < K && k2 > k1 && k2 < K) {
-            const long long e = first(k1, K) + (k2 - k1 - 1);
-            unsigned pk;
-            double un;
-            if (INIT) {
-                const int a = cl1[e], c2 = cl2[e];
-                const bool ok = a >= 0 && a <= 2 * L && c2 >= 0 && c2 <= 2 * L;
-                pk = ok ? ((unsigned)a | ((unsigned)c2 << 16)) : (z | (z << 16));
-                pack[e] = pk;
-                un = b[e];
+    // rows in groups of UR, double-buffered in registers: the next group's
+    // loads are issued before this group's stores (u is read and written in
+    // place, so the compiler cannot move a later row's load above an
+    // earlier row's store); the rows are finished in order
+    constexpr int UR = 4;
+    // pack and u are stored tile by tile (TR x TC slots per tile, row-major):
+    // a workgroup streams one contiguous block instead of TR rows that lie
+    // K entries apart in equation order
+    const size_t tbase = (size_t)blockIdx.x * TR * TC + t;
+    unsigned pk[2][UR];
+    double ub[2][UR];
+    // The first pass writes every slot of a tile: a pair's packed lines and
+    // right-hand side, or the no-line code (zero coefficients) and u = 0
+    // where the tile holds no pair.  The later passes then load, compute
+    // and store every slot without a branch (a slot without a pair stays 0
+    // and adds +0 to every sum), which lets the compiler count the loads in
+    // flight and wait for exactly the row it finishes.
+    auto load_group = [&](int r0, unsigned* gp, double* gu) {
+#pragma unroll
+        for (int i = 0; i < UR; i++) {
+            if constexpr (INIT) {
+                const int k1 = TR * kb + r0 + i;
+                gp[i] = z | (z << 16);
+                gu[i] = 0.0;
+                if (k1 < K && k2 > k1 && k2 < K) {
+                    const long long e = first(k1, K) + (k2 - k1 - 1);
+                    const int a = cl1[e], c2 = cl2[e];
+                    if (a >= 0 && a <= 2 * L && c2 >= 0 && c2 <= 2 * L) gp[i] = (unsigned)a | ((unsigned)c2 << 16);
+                    gu[i] = b[e];
+                }
             } else {
-                pk = pack[e];
+                const size_t q = tbase + (size_t)(r0 + i) * TC;
+                gp[i] = pack[q];
+                gu[i] = u[q];
             }
-            const double2 A = ta[pk & 0xffff];
-            const double* B = tb + 3 * (pk >> 16);
-            const double sg = B[2];
-            if (!INIT) {
-                const double av = sg * A.x * (v[2 * k1] * vs) + sg * A.y * (v[2 * k1 + 1] * vs) + B[0] * va0 +
-                                  B[1] * va1;
-                un = av - ca * u[e];
+        }
+    };
+    auto finish_group = [&](int r0, const unsigned* gp, const double* gu) {
+#pragma unroll
+        for (int i = 0; i < UR; i++) {
+            const int r = r0 + i;
+            const size_t q = tbase + (size_t)r * TC;
+            double2 A;
+            double B0, B1, sg;
+            if constexpr (LT) {
+                const int ia = gp[i] & 0xffff, ib = 2 * (2 * L + 2) + 3 * (gp[i] >> 16);
+                A = make_double2(s_tab[2 * ia], s_tab[2 * ia + 1]);
+                B0 = s_tab[ib];
+                B1 = s_tab[ib + 1];
+                sg = s_tab[ib + 2];
+            } else {
+                A = ta[gp[i] & 0xffff];
+                const double* B = tb + 3 * (gp[i] >> 16);
+                B0 = B[0];
+                B1 = B[1];
+                sg = B[2];
             }
-            u[e] = un;
+            double un;
+            if constexpr (INIT) {
+                pack[q] = gp[i];
+                un = gu[i];
+            } else {
+                const double av = sg * A.x * (s_v[2 * r] * vs) + sg * A.y * (s_v[2 * r + 1] * vs) + B0 * va0 +
+                                  B1 * va1;
+                un = av - ca * gu[i];
+            }
+            u[q] = un;
             sq += un * un;
-            r0 = sg * A.x * un;
-            r1 = sg * A.y * un;
-            c0 += B[0] * un;
-            c1 += B[1] * un;
+            double r0v = sg * A.x * un;
+            double r1v = sg * A.y * un;
+            c0 += B0 * un;
+            c1 += B1 * un;
+            wave_sum2(r0v, r1v);
+            if (lane == 0) {
+                s_row[r][wave][0] = r0v;
+                s_row[r][wave][1] = r1v;
+            }
         }
-        r0 = wave_sum(r0);
-        r1 = wave_sum(r1);
-        if (lane == 0) {
-            s_row[r][wave][0] = r0;
-            s_row[r][wave][1] = r1;
-        }
+    };
+    static_assert((TR / UR) % 2 == 0, "groups come in pairs");
+    load_group(0, pk[0], ub[0]);
+#pragma unroll 1
+    for (int r0 = 0; r0 < TR; r0 += 2 * UR) {
+        load_group(r0 + UR, pk[1], ub[1]);
+        finish_group(r0, pk[0], ub[0]);
+        if (r0 + 2 * UR < TR) load_group(r0 + 2 * UR, pk[0], ub[0]);
+        finish_group(r0 + UR, pk[1], ub[1]);
     }
     sq = wave_sum(sq);
     if (lane == 0) s_sq[wave] = sq;
@@ -251,29 +357,88 @@ __global__ __launch_bounds__(1024) void beta_kernel(int n, const double* part, S
     }
 }
 
-// Group g's share (g = 0..3) of (A'u)[i], entry i of the 2K-vector: the
+// Entries of the 2K-vector per workgroup of the element-parallel steps: 16
+// entries x 16 partial groups (256 threads); the first 16 threads finish the
+// entries.
+constexpr int VE = 16;
+constexpr int NG = 256 / VE;  // partial groups per entry
+
+// Group g's share (g = 0..NG-1) of (A'u)[i], entry i of the 2K-vector: the
 // row parts of the tiles in image i/2's row block, then the column parts of
-// the tiles in its column block, each in block order over the g-th quarter
-// of the blocks.  (A'u)[i] = ((g0 + g1) + g2) + g3.
+// the tiles in its column block, each in block order over the g-th of NG
+// slices of the blocks.  (A'u)[i] = (..((g0 + g1) + g2) ..) + g(NG-1).
 __device__ inline double atu_part(int K, const double* rowp, const double* colp, int i, int g) {
     const int k = i >> 1, n2 = 2 * K;
     const int kb = k / TR, cbk = k / TC;
     const int ncb = (K + TC - 1) / TC, nkb = (K + TR - 1) / TR;
+    // the partials of tiles that do not exist are zero (cleared before the
+    // solve, never written), so every block is added: +0 changes no sum
+    // that starts at +0, and the loads need no branch
+    (void)kb;
+    (void)cbk;
     double t = 0.0;
-    for (int cb = g * ncb / 4; cb < (g + 1) * ncb / 4; cb++)
-        if (tile_exists(K, kb, cb)) t += rowp[(size_t)cb * n2 + i];
-    for (int b = g * nkb / 4; b < (g + 1) * nkb / 4; b++)
-        if (tile_exists(K, b, cbk)) t += colp[(size_t)b * n2 + i];
+    const double* rp = rowp + (size_t)(g * ncb / NG) * n2 + i;
+#pragma unroll 4
+    for (int cb = g * ncb / NG; cb < (g + 1) * ncb / NG; cb++, rp += n2) t += *rp;
+    const double* cp = colp + (size_t)(g * nkb / NG) * n2 + i;
+#pragma unroll 4
+    for (int b = g * nkb / NG; b < (g + 1) * nkb / NG; b++, cp += n2) t += *cp;
     return t;
 }
 
-// Entries of the 2K-vector per workgroup of the element-parallel steps: 64
-// entries x 4 partial groups (256 threads); wave 0 finishes the entries.
-constexpr int VE = 64;
+
+// One LSQR iteration after the pass is two launches:
+//  stepa_kernel (NG threads per element): the previous step's update of x
+//    and w, v <- A'u / beta - beta v, per-workgroup partial norms;
+//  stepb_kernel, one wave (scalar_step): the running
+//    Frobenius estimate of A, alpha = |v|, the damping and plane rotations,
+//    the stopping tests; the update's coefficients and the normalization
+//    1/alpha of v go to the state, applied by the next pass and stepa (or
+//    finish_kernel after the last iteration).
+// beta = 0 (b in the range of A): v and alpha stay, the rotation makes the
+// last update of x and phibar becomes 0.
+// Run by one whole wave (lane 0 updates the state).
+__device__ inline void scalar_step(double beta, int nb, const double* part, State* S) {
+    double alpha = S->alpha;
+    if (beta > 0.0) alpha = sqrt(wave_fixed_sum(part, nb));
+    if (threadIdx.x == 0) {
+        S->beta = beta;
+        if (beta > 0.0) S->anorm = sqrt(S->anorm * S->anorm + S->alpha * S->alpha + beta * beta + S->damp * S->damp);
+        const double dampsq = S->damp * S->damp;
+        const double rhobar1 = sqrt(S->rhobar * S->rhobar + dampsq);
+        const double cs1 = S->rhobar / rhobar1, sn1 = S->damp / rhobar1;
+        const double psi = sn1 * S->phibar;
+        const double phibar1 = cs1 * S->phibar;
+        const double rho = sqrt(rhobar1 * rhobar1 + beta * beta);
+        const double cs = rhobar1 / rho, sn = beta / rho;
+        const double theta = sn * alpha;
+        S->rhobar = -cs * alpha;
+        const double phi = cs * phibar1;
+        S->phibar = sn * phibar1;
+        const double tau = sn * phi;
+        S->t1 = phi / rho;
+        S->t2 = -theta / rho;
+        S->vscale = beta > 0.0 ? (alpha > 0.0 ? 1.0 / alpha : 0.0) : 1.0;
+        S->upd = 1;
+        S->alpha = alpha;
+        S->res2 += psi * psi;
+        S->rnorm = sqrt(S->phibar * S->phibar + S->res2);
+        S->arnorm = alpha * fabs(tau);
+        S->itn += 1;
+        if (S->rnorm <= S->tol * S->bnorm || S->arnorm <= S->tol * S->anorm * S->rnorm) {
+            S->done = 1;
+            S->flag = 0;
+        } else if (S->itn >= S->maxit) {
+            S->done = 1;
+            S->flag = 1;
+        }
+    }
+}
 
 // INIT (first direction, u = b/beta, beta from beta_kernel): v <- A'u / beta.
-// Else, with beta = |u| from the tiles' sums of squares (fixed_sum, the
-// same bits in every workgroup) and the current v (stored v times vscale):
+// Else, with beta = |u| from the tiles' sums of squares (wave_fixed_sum,
+// the same bits in every workgroup) and the current v (stored v times
+// vscale):
 // the last scalar step's pending update of x and w, then v <- A'u / beta -
 // beta v (v unchanged when beta = 0).  Both with per-workgroup partial sums
 // of the squares of the new v (fixed order).
@@ -281,19 +446,25 @@ template <bool INIT>
 __global__ __launch_bounds__(256) void stepa_kernel(int K, int ntiles, const double* normp, const double* rowp,
                                                     const double* colp, const State* S, double* v, double* w,
                                                     double* x, double* part) {
-    __shared__ double sh[4][VE];
-    __shared__ double sb[256];
+    __shared__ double sh[NG][VE];
+    __shared__ double s_beta;
     const int n2 = 2 * K;
     if (S->done) return;
-    const double beta = INIT ? S->beta : sqrt(fixed_sum<256>(normp, ntiles, sb));
-    const int j = threadIdx.x & (VE - 1), g = threadIdx.x / VE;
+    if (!INIT && threadIdx.x < 64) {
+        const double bb = sqrt(wave_fixed_sum(normp, ntiles));  // the bits stepb_kernel gets
+        if (threadIdx.x == 0) s_beta = bb;
+    }
+    const int j = threadIdx.x % VE, g = threadIdx.x / VE;
     const int i = blockIdx.x * VE + j;
-    sh[g][j] = i < n2 && beta > 0.0 ? atu_part(K, rowp, colp, i, g) : 0.0;
+    sh[g][j] = i < n2 ? atu_part(K, rowp, colp, i, g) : 0.0;  // used only when beta > 0
     __syncthreads();
-    if (g == 0) {
+    const double beta = INIT ? S->beta : s_beta;
+    if (threadIdx.x < 64) {
         double t = 0.0;
-        if (i < n2) {
-            const double a = ((sh[0][j] + sh[1][j]) + sh[2][j]) + sh[3][j];
+        if (threadIdx.x < VE && i < n2) {
+            double a = sh[0][j];
+#pragma unroll
+            for (int q = 1; q < NG; q++) a += sh[q][j];
             if (INIT) {
                 t = a * (1.0 / beta);
             } else {
@@ -308,7 +479,7 @@ __global__ __launch_bounds__(256) void stepa_kernel(int K, int ntiles, const dou
             v[i] = t;
         }
         const double sq = wave_sum(t * t);
-        if (j == 0) part[blockIdx.x] = sq;
+        if (threadIdx.x == 0) part[blockIdx.x] = sq;
     }
 }
 
@@ -354,60 +525,12 @@ __global__ __launch_bounds__(256) void vinit_c_kernel(int n2, const State* S, do
     }
 }
 
-// One LSQR iteration after the pass, in two launches:
-//  stepa (above; 4 threads per element): the previous step's update of x
-//    and w, v <- A'u / beta - beta v, per-workgroup partial norms;
-//  stepb (one workgroup): beta = |u| (the same sum as stepa's) and the
-//    running Frobenius estimate of A, alpha = |v|, the damping and plane
-//    rotations, the stopping tests; the update's coefficients and the
-//    normalization 1/alpha of v go to the state, applied by the next
-//    pass and stepa (or finish_kernel after the last iteration).
-// beta = 0 (b in the range of A): v and alpha stay, the rotation below makes
-// the last update of x and phibar becomes 0.
-__global__ __launch_bounds__(1024) void stepb_kernel(int ntiles, const double* normp, int nb, const double* part,
-                                                     State* S) {
-    __shared__ double sh[1024];
+// The scalar step, one wave: beta = |u| (the same sum as stepa's), then
+// scalar_step on stepa's per-workgroup sums of squares.
+__global__ __launch_bounds__(64) void stepb_kernel(int ntiles, const double* normp, int nb, const double* part,
+                                                   State* S) {
     if (S->done) return;
-    const double beta = sqrt(fixed_sum<1024>(normp, ntiles, sh));
-    double alpha = S->alpha;
-    if (beta > 0.0) {
-        double sq = 0.0;
-        for (int i = threadIdx.x; i < nb; i += 1024) sq += part[i];
-        sq = block_sum<1024>(sq, sh);
-        alpha = sqrt(sq);
-    }
-    if (threadIdx.x == 0) {
-        S->beta = beta;
-        if (beta > 0.0) S->anorm = sqrt(S->anorm * S->anorm + S->alpha * S->alpha + beta * beta + S->damp * S->damp);
-        const double dampsq = S->damp * S->damp;
-        const double rhobar1 = sqrt(S->rhobar * S->rhobar + dampsq);
-        const double cs1 = S->rhobar / rhobar1, sn1 = S->damp / rhobar1;
-        const double psi = sn1 * S->phibar;
-        const double phibar1 = cs1 * S->phibar;
-        const double rho = sqrt(rhobar1 * rhobar1 + beta * beta);
-        const double cs = rhobar1 / rho, sn = beta / rho;
-        const double theta = sn * alpha;
-        S->rhobar = -cs * alpha;
-        const double phi = cs * phibar1;
-        S->phibar = sn * phibar1;
-        const double tau = sn * phi;
-        S->t1 = phi / rho;
-        S->t2 = -theta / rho;
-        S->vscale = beta > 0.0 ? (alpha > 0.0 ? 1.0 / alpha : 0.0) : 1.0;
-        S->upd = 1;
-        S->alpha = alpha;
-        S->res2 += psi * psi;
-        S->rnorm = sqrt(S->phibar * S->phibar + S->res2);
-        S->arnorm = alpha * fabs(tau);
-        S->itn += 1;
-        if (S->rnorm <= S->tol * S->bnorm || S->arnorm <= S->tol * S->anorm * S->rnorm) {
-            S->done = 1;
-            S->flag = 0;
-        } else if (S->itn >= S->maxit) {
-            S->done = 1;
-            S->flag = 1;
-        }
-    }
+    scalar_step(sqrt(wave_fixed_sum(normp, ntiles)), nb, part, S);
 }
 
 // After the last iteration: its pending update of x (x += t1 w).
@@ -451,7 +574,8 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     }
     const int ntiles = c->ls_ntiles;
     const int nstep = (n2 + VE - 1) / VE, nelem = (n2 + 255) / 256;
-    if ((rc = ensure(c, c->ls_pack, (size_t)P * 4)) || (rc = ensure(c, c->ls_u, (size_t)P * 8)) ||
+    const size_t nslot = (size_t)ntiles * TR * TC;  // pack and u, tile by tile
+    if ((rc = ensure(c, c->ls_pack, nslot * 4)) || (rc = ensure(c, c->ls_u, nslot * 8)) ||
         (rc = ensure(c, c->ls_part, (size_t)(ntiles + nstep) * 8)) ||
         (rc = ensure(c, c->ls_vec, (size_t)n2 * (2 + ncb + nkb) * 8)) ||
         (rc = ensure(c, c->ls_tab, (size_t)(2 * L + 2) * 5 * 8)) || (rc = ensure(c, c->ls_state, sizeof(State))))
@@ -475,8 +599,30 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, hipMemcpyAsync(S, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     tables_kernel<<<(2 * L + 2 + 255) / 256, 256, 0, c->stream>>>(L, ta, tb);
-    pass_kernel<true><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, d_cl1, d_cl2, d_shift1d, pack, ta, tb, v, S, u, part,
-                                                     rowp, colp);
+    // coefficient tables in LDS when they are small (n_theta up to 816)
+    const size_t tab_bytes = (size_t)5 * (2 * L + 2) * 8;
+    const bool lt = tab_bytes <= 32768;
+    auto pass = [&](bool init) {
+        if (init) {
+            if (lt)
+                pass_kernel<true, true><<<ntiles, 256, tab_bytes, c->stream>>>(K, L, tiles, d_cl1, d_cl2, d_shift1d,
+                                                                               pack, ta, tb, v, S, u, part, rowp, colp);
+            else
+                pass_kernel<true, false><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, d_cl1, d_cl2, d_shift1d, pack, ta,
+                                                                        tb, v, S, u, part, rowp, colp);
+        } else {
+            if (lt)
+                pass_kernel<false, true><<<ntiles, 256, tab_bytes, c->stream>>>(K, L, tiles, nullptr, nullptr, nullptr,
+                                                                                pack, ta, tb, v, S, u, part, rowp,
+                                                                                colp);
+            else
+                pass_kernel<false, false><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, nullptr, nullptr, nullptr, pack,
+                                                                         ta, tb, v, S, u, part, rowp, colp);
+        }
+    };
+    // partials of the tiles that do not exist stay zero (stepa adds every block)
+    HIPCHK(c, hipMemsetAsync(rowp, 0, (size_t)(ncb + nkb) * n2 * 8, c->stream));
+    pass(true);
     beta_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, S);
     stepa_kernel<true><<<nstep, 256, 0, c->stream>>>(K, ntiles, part, rowp, colp, S, v, w, d_x, part2);
     vinit_b_kernel<<<1, 1024, 0, c->stream>>>(nstep, part2, S);
@@ -485,10 +631,9 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     // launches after the stopping tests pass return at once; the done flag is
     // read back every 8 iterations so a converged solve stops issuing them
     for (int it = 0; it < maxit; it++) {
-        pass_kernel<false><<<ntiles, 256, 0, c->stream>>>(K, L, tiles, nullptr, nullptr, nullptr, pack, ta, tb, v, S, u,
-                                                          part, rowp, colp);
+        pass(false);
         stepa_kernel<false><<<nstep, 256, 0, c->stream>>>(K, ntiles, part, rowp, colp, S, v, w, d_x, part2);
-        stepb_kernel<<<1, 1024, 0, c->stream>>>(ntiles, part, nstep, part2, S);
+        stepb_kernel<<<1, 64, 0, c->stream>>>(ntiles, part, nstep, part2, S);
         if (it % 8 == 7 && it + 1 < maxit) {
             int done = 0;
             HIPCHK(c, hipGetLastError());
