@@ -20,7 +20,8 @@
 // and, in the same pass, A'u' (A' is linear, so the normalization by the new
 // beta = |u'| is applied afterwards to the 2K-vector).  Memory per iteration
 // (K = 5000, 12.5M pairs): the packed lines (4 B/pair) and u (8 B/pair) are
-// read once and u written once, 20 B/pair, about 0.25 GB, plus the 2K-long
+// read once and u written once, 20 B/pair, about 0.25 GB (stored tile by
+// tile, with the empty slots of the diagonal tiles), plus the 2K-long
 // partial sums of the 64 x 256-pair tiles; HBM-bound work with every
 // reduction in a fixed order (results do not depend on scheduling).
 #include <hip/hip_runtime.h>
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(256) void pass_kernel(int K, int L, const int2* __r
                                                    const int* __restrict__ cl1, const int* __restrict__ cl2,
                                                    const double* __restrict__ b, unsigned* __restrict__ pack,
                                                    const double2* __restrict__ ta, const double* __restrict__ tb,
-                                                   const double* __restrict__ v, State* S, double* u,
+                                                   const double* __restrict__ v, const State* S, double* u,
                                                    double* normp, double* rowp, double* colp) {
     __shared__ double s_row[TR][4][2];
     __shared__ double s_sq[4];
