@@ -207,14 +207,19 @@ __device__ inline void f_both(double x, const double* prm, const double* lg, dou
     ind = hi == -INFINITY ? hi : hi + log1p(exp(lo - hi));
 }
 
-template <int MODE>
+// NEC: vectors the signs product stages (a.ne <= NEC); its third images are
+// staged 12 at a time (43 KB of LDS at NEC = 4: three workgroups per CU),
+// the other modes' 16 at a time.  The stage size does not change the order
+// of any sum.
+template <int MODE, int NEC = 1>
 __device__ __forceinline__ void tri_body(TriArgs a) {
-    __shared__ double Lp[9][KC][TB];
-    __shared__ double Lq[9][KC][TB];
-    __shared__ double Vp[MODE == SIGNS ? MAXE : 1][KC][TB];
-    __shared__ double Vq[MODE == SIGNS ? MAXE : 1][KC][TB];
-    __shared__ double Sp[MODE == SIGNS ? KC : 1][TB];
-    __shared__ double Sq[MODE == SIGNS ? KC : 1][TB];
+    constexpr int KCm = MODE == SIGNS ? 12 : KC;
+    __shared__ double Lp[9][KCm][TB];
+    __shared__ double Lq[9][KCm][TB];
+    __shared__ double Vp[MODE == SIGNS ? NEC : 1][KCm][TB];
+    __shared__ double Vq[MODE == SIGNS ? NEC : 1][KCm][TB];
+    __shared__ double Sp[MODE == SIGNS ? KCm : 1][TB];
+    __shared__ double Sq[MODE == SIGNS ? KCm : 1][TB];
     extern __shared__ unsigned int s_hist[];
     const int N = a.N;
     const long long NN = (long long)N * N;
@@ -231,28 +236,28 @@ __device__ __forceinline__ void tri_body(TriArgs a) {
 #pragma unroll
         for (int e = 0; e < 9; e++) Rpq[e] = act ? a.Rs[e * NN + (long long)p * N + q] : 0.0;
         double acc0 = 0.0, acc1 = 0.0;
-        double vacc[MODE == SIGNS ? MAXE : 1];
+        double vacc[MODE == SIGNS ? NEC : 1];
 #pragma unroll
-        for (int g = 0; g < (MODE == SIGNS ? MAXE : 1); g++) vacc[g] = 0.0;
-        for (int k0 = 0; k0 < N; k0 += KC) {
+        for (int g = 0; g < (MODE == SIGNS ? NEC : 1); g++) vacc[g] = 0.0;
+        for (int k0 = 0; k0 < N; k0 += KCm) {
             __syncthreads();
-            // stage Rs[e][k][p0..p0+15] and Rs[e][k][q0..q0+15] for k in [k0, k0+KC)
-            for (int x = tid; x < 9 * KC * TB; x += blockDim.x) {
-                const int e = x / (KC * TB), kk = (x / TB) % KC, j = x % TB;
+            // stage Rs[e][k][p0..p0+15] and Rs[e][k][q0..q0+15] for k in [k0, k0+KCm)
+            for (int x = tid; x < 9 * KCm * TB; x += blockDim.x) {
+                const int e = x / (KCm * TB), kk = (x / TB) % KCm, j = x % TB;
                 const int k = k0 + kk;
                 const bool okp = k < N && p0 + j < N, okq = k < N && q0 + j < N;
                 Lp[e][kk][j] = okp ? a.Rs[e * NN + (long long)k * N + p0 + j] : 0.0;
                 Lq[e][kk][j] = okq ? a.Rs[e * NN + (long long)k * N + q0 + j] : 0.0;
             }
             if constexpr (MODE == SIGNS) {
-                for (int x = tid; x < a.ne * KC * TB; x += blockDim.x) {
-                    const int g = x / (KC * TB), kk = (x / TB) % KC, j = x % TB;
+                for (int x = tid; x < a.ne * KCm * TB; x += blockDim.x) {
+                    const int g = x / (KCm * TB), kk = (x / TB) % KCm, j = x % TB;
                     const int k = k0 + kk;
                     Vp[g][kk][j] = (k < N && p0 + j < N) ? a.Vs[g * NN + (long long)k * N + p0 + j] : 0.0;
                     Vq[g][kk][j] = (k < N && q0 + j < N) ? a.Vs[g * NN + (long long)k * N + q0 + j] : 0.0;
                 }
                 if (a.entries == 2)
-                    for (int x = tid; x < KC * TB; x += blockDim.x) {
+                    for (int x = tid; x < KCm * TB; x += blockDim.x) {
                         const int kk = x / TB, j = x % TB, k = k0 + kk;
                         Sp[kk][j] = (k < N && p0 + j < N) ? a.Ss[(long long)k * N + p0 + j] : 0.0;
                         Sq[kk][j] = (k < N && q0 + j < N) ? a.Ss[(long long)k * N + q0 + j] : 0.0;
@@ -260,7 +265,7 @@ __device__ __forceinline__ void tri_body(TriArgs a) {
             }
             __syncthreads();
             if (!act) continue;
-            const int kend = min(KC, N - k0);
+            const int kend = min(KCm, N - k0);
             for (int kk = 0; kk < kend; kk++) {
                 const int m = k0 + kk;
                 if (m == p || m == q) continue;
@@ -320,7 +325,7 @@ __device__ __forceinline__ void tri_body(TriArgs a) {
                     // ij: v2 += e(ij,jk) v[jk] + e(ij,ik) v[ik]; jk: e(ij,jk) v[ij] + e(ik,jk) v[ik];
                     // ik: e(ij,ik) v[ij] + e(ik,jk) v[jk]
 #pragma unroll
-                    for (int g = 0; g < MAXE; g++) {
+                    for (int g = 0; g < NEC; g++) {
                         if (g >= a.ne) break;
                         const double vpm = Vp[g][kk][tp], vqm = Vq[g][kk][tq];
                         const double x = role == 0 ? ea * vqm : (role == 1 ? ea * vpm : eb * vpm);
@@ -339,7 +344,7 @@ __device__ __forceinline__ void tri_body(TriArgs a) {
                 a.arb[e] = acc1;
             } else {
 #pragma unroll
-                for (int g = 0; g < MAXE; g++)
+                for (int g = 0; g < NEC; g++)
                     if (g < a.ne) a.v2[e + a.P * g] = vacc[g];
             }
         }
@@ -362,11 +367,13 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 }
 
 // One kernel per mode, for their own occupancy: the scores fit 128 VGPRs
-// (four waves per SIMD); the probabilities (log, exp, log1p) and the signs
-// product (staged vectors) need more and run at two.
+// (four waves per SIMD); the probabilities (log, exp, log1p) need more and
+// run at two; the signs product runs at three for up to four vectors (its
+// LDS and registers sized for them), at two for up to eight.
 __global__ __launch_bounds__(256, 4) void tri_scores_kernel(TriArgs a) { tri_body<SCORES>(a); }
 __global__ __launch_bounds__(256) void tri_probs_kernel(TriArgs a) { tri_body<PROBS>(a); }
-__global__ __launch_bounds__(256) void tri_signs_kernel(TriArgs a) { tri_body<SIGNS>(a); }
+__global__ __launch_bounds__(256, 3) void tri_signs4_kernel(TriArgs a) { tri_body<SIGNS, 4>(a); }
+__global__ __launch_bounds__(256) void tri_signs8_kernel(TriArgs a) { tri_body<SIGNS, MAXE>(a); }
 
 }  // namespace s3
 
@@ -432,8 +439,10 @@ int s3_launch(clm_ctx* c, const s3::TriArgs& a, size_t dyn_lds) {
         s3::tri_scores_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
     else if constexpr (MODE == s3::PROBS)
         s3::tri_probs_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
+    else if (a.ne <= 4)
+        s3::tri_signs4_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
     else
-        s3::tri_signs_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
+        s3::tri_signs8_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
     HIPCHK(c, hipGetLastError());
     return CLM_OK;
 }
