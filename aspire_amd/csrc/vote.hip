@@ -146,6 +146,23 @@ __device__ inline void pair_of(long long e, int K, int& k1, int& k2) {
 // no triangle voted.  A kernel of its own, so that its few registers allow
 // many waves per SIMD: the loop is a chain of dependent FP64 operations per
 // angle, and occupancy hides its latency.
+// t / 18 correctly rounded, without the general division sequence.  Let
+// q = t/18 lie in [2^e, 2^(e+1)) and u = 2^(e-52) (its ulp).  R = RN(1/18)
+// has |R - 1/18| <= 2^-58, so q0 = RN(t R) is within 1.6 u of q; the
+// remainder r = t - 18 q0 is a multiple of u (t is a multiple of 16 u, 18 q0
+// of 9 u) below 29 u in magnitude, so the FMA computes it exactly; and
+// q0 + r R, rounded once by the second FMA, differs from q by
+// |r| |R - 1/18| < 29 u 2^-58.  q itself lies at least u/18 from every
+// rounding midpoint (t - 18 (2k+1) u/2 is an odd multiple of u), so both
+// round to the same double.  t here is 0 or a normal number (a combination
+// of angles in degrees), far from the subnormal range.
+__device__ inline double div18(double t) {
+    constexpr double R = 1.0 / 18.0;
+    const double q0 = t * R;
+    const double r = fma(-q0, 18.0, t);
+    return fma(r, R, q0);
+}
+
 __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const unsigned short* M,
                                                    const unsigned short* MT, const double2* cs_g, int* peak,
                                                    int cs_lds) {
@@ -177,7 +194,7 @@ __global__ __launch_bounds__(256) void peak_kernel(int K, int n_theta, const uns
             double t = 2 * a * x;
             t = t - a * a;
             t = t - x * x;
-            return exp(t / 18.0);
+            return exp(div18(t));
         };
         for (int base = 64 * wave; base < K; base += 256) {
             const int k3 = base + lane;
