@@ -64,6 +64,22 @@ def test_iterates_match_oracle(iters, damp):
     assert abs(st["relres"] - io["relres"]) <= 1e-9 * io["relres"]
 
 
+@pytest.mark.parametrize("K,n_theta", [(300, 36), (70, 1000)])
+def test_tile_edges_and_global_tables(K, n_theta):
+    # K = 300: a last row block of 44 images and a partial column block
+    # (64 x 256 tiles); n_theta = 1000: coefficient tables too large for
+    # LDS (5 (n_theta + 2) doubles > 32 KB), read from global memory
+    say("case lsqr-tiles", K, n_theta)
+    cl1, cl2, b = random_case(K, n_theta, seed=K + n_theta)
+    A, _ = so.system(K, n_theta, cl1, cl2, b)
+    xo, io = so.lsqr(A, b, tol=0.0, maxit=12, damp=0.0)
+    x, st = estimate_shifts(*dev(cl1, cl2, b), K, n_theta, tol=0.0, maxit=12, damp=0.0)
+    x = x.cpu().numpy().ravel()
+    say(st, io)
+    assert st["iterations"] == io["iterations"] == 12
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-10 * np.abs(xo).max())
+
+
 @pytest.mark.parametrize("consistent", [False, True])
 def test_reference_settings_match_oracle(consistent):
     say("case lsqr-default", consistent)
