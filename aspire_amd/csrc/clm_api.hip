@@ -1977,75 +1977,72 @@ int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, const i
     const long long ncol = 2 * (long long)K;
     const std::vector<int32_t> rank = selected_rank(P, sel);
     auto row_of = [&](long long e) -> long long { return sel ? (long long)rank[(size_t)e] : e; };
-    // column q = 2k + xy (image k, coordinate xy): rows where k is the second
-    // image (pairs (j, k), j < k), then rows where it is the first (pairs
-    // (k, m), m > k), nonzero coefficients only (MATLAB's sparse keeps no
-    // zeros); column 2K: the right-hand side.  Columns are counted, then
-    // filled, by host threads; the result does not depend on their number.
-    auto column = [&](long long q, int64_t* out_ir, double* out_pr) -> long long {
-        const long long k = q >> 1;
-        const int xy = (int)(q & 1);
-        long long n = 0;
+    // The nonzeros in pair order: pair e = (k1, k2) has its coefficients in
+    // columns 2k1 + xy (c4[xy]) and 2k2 + xy (c4[2 + xy]) and, when its 1-D
+    // shift is nonzero, an entry in column 2K (the right-hand side).  Walking
+    // the pairs in order appends every column's rows in increasing order, as
+    // the compressed-column form needs.  Chunks of consecutive pairs are
+    // counted per column, then filled from offsets that follow the chunk
+    // order, so the result does not depend on the number of host threads.
+    const long long nch = P == 0 ? 0 : std::min<long long>(std::max<long long>(P / 262144, 1), 256);
+    const long long W = ncol + 1;
+    std::vector<long long> cnt((size_t)(nch * W), 0);
+    auto first_of = [&](long long k) { return k * K - k * (k + 1) / 2; };
+    auto walk = [&](long long c, auto&& emit) {
+        const long long e0 = P * c / nch, e1 = P * (c + 1) / nch;
+        if (e0 >= e1) return;
+        const double t = 2.0 * K - 1.0;
+        long long k1 = (long long)std::floor((t - std::sqrt(std::max(t * t - 8.0 * (double)e0, 0.0))) / 2.0);
+        k1 = std::min<long long>(std::max<long long>(k1, 0), K - 2);
+        while (k1 > 0 && first_of(k1) > e0) k1--;
+        while (k1 + 1 <= K - 2 && first_of(k1 + 1) <= e0) k1++;
+        long long k2 = e0 - first_of(k1) + k1 + 1;
         double c4[4];
-        for (long long j = 0; j < k; j++) {
-            const long long e = pair_slot(j, k, K);
-            if (sel && !sel[e]) continue;
-            coefs(e, c4);
-            const double v = c4[2 + xy];
-            if (v != 0.0) {
-                if (out_ir) { out_ir[n] = row_of(e); out_pr[n] = v; }
-                n++;
+        for (long long e = e0; e < e1; e++) {
+            if (!sel || sel[e]) {
+                const long long row = row_of(e);
+                coefs(e, c4);
+                for (int xy = 0; xy < 2; xy++)
+                    if (c4[xy] != 0.0) emit(2 * k1 + xy, row, c4[xy]);
+                for (int xy = 0; xy < 2; xy++)
+                    if (c4[2 + xy] != 0.0) emit(2 * k2 + xy, row, c4[2 + xy]);
+                if (shift1d[e] != 0.0) emit(ncol, row, shift1d[e]);
+            }
+            if (++k2 == K) {
+                k1++;
+                k2 = k1 + 1;
             }
         }
-        for (long long m = k + 1; m < K; m++) {
-            const long long e = pair_slot(k, m, K);
-            if (sel && !sel[e]) continue;
-            coefs(e, c4);
-            const double v = c4[xy];
-            if (v != 0.0) {
-                if (out_ir) { out_ir[n] = row_of(e); out_pr[n] = v; }
-                n++;
-            }
-        }
-        return n;
     };
-    std::vector<long long> cnt(ncol + 1, 0);
-    host_parallel(ncol, 65536 / std::max(K, 1) + 1, [&](long long q0, long long q1) {
-        for (long long q = q0; q < q1; q++) cnt[q] = column(q, nullptr, nullptr);
-    });
-    // right-hand side, counted in blocks
-    const long long nblk = std::min<long long>(std::max<long long>(P / 65536, 1), 4096);
-    std::vector<long long> rcnt(nblk, 0);
-    host_parallel(nblk, 1, [&](long long b0, long long b1) {
-        for (long long b = b0; b < b1; b++) {
-            long long n = 0;
-            for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++) n += shift1d[e] != 0.0 && (!sel || sel[e]);
-            rcnt[b] = n;
+    host_parallel(nch, 1, [&](long long c0, long long c1) {
+        for (long long c = c0; c < c1; c++) {
+            long long* cc = cnt.data() + c * W;
+            walk(c, [&](long long col, long long, double) { cc[col]++; });
         }
     });
-    std::vector<long long> start(ncol + 2, 0);
-    for (long long q = 0; q < ncol; q++) start[q + 1] = start[q] + cnt[q];
-    long long rtot = 0;
-    for (long long b = 0; b < nblk; b++) rtot += rcnt[b];
-    start[ncol + 1] = start[ncol] + rtot;
+    // column starts, and each chunk's offset within every column
+    std::vector<long long> start((size_t)W + 1, 0);
+    for (long long col = 0; col < W; col++) {
+        long long n = start[col];
+        for (long long c = 0; c < nch; c++) {
+            const long long k = cnt[c * W + col];
+            cnt[c * W + col] = n;
+            n += k;
+        }
+        start[col + 1] = n;
+    }
     if (jc)
         for (long long q = 0; q <= ncol + 1; q++) jc[q] = start[q];
-    if (nnz) *nnz = start[ncol + 1];
+    if (nnz) *nnz = start[W];
     if (ir && pr) {
-        host_parallel(ncol, 65536 / std::max(K, 1) + 1, [&](long long q0, long long q1) {
-            for (long long q = q0; q < q1; q++) column(q, ir + start[q], pr + start[q]);
-        });
-        std::vector<long long> rstart(nblk, start[ncol]);
-        for (long long b = 1; b < nblk; b++) rstart[b] = rstart[b - 1] + rcnt[b - 1];
-        host_parallel(nblk, 1, [&](long long b0, long long b1) {
-            for (long long b = b0; b < b1; b++) {
-                long long n = rstart[b];
-                for (long long e = P * b / nblk; e < P * (b + 1) / nblk; e++)
-                    if (shift1d[e] != 0.0 && (!sel || sel[e])) {
-                        ir[n] = row_of(e);
-                        pr[n] = shift1d[e];
-                        n++;
-                    }
+        host_parallel(nch, 1, [&](long long c0, long long c1) {
+            for (long long c = c0; c < c1; c++) {
+                long long* off = cnt.data() + c * W;
+                walk(c, [&](long long col, long long row, double v) {
+                    const long long i = off[col]++;
+                    ir[i] = row;
+                    pr[i] = v;
+                });
             }
         });
     }
