@@ -267,6 +267,10 @@ def assemble(K: int, n_theta: int, rec, sel=None):
     rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), vp(ir), vp(pr), ctypes.byref(nnz))
     if rc:
         raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
+    if nrow < 2 ** 31 and nnz.value < 2 ** 31:
+        # scipy keeps 32-bit indices when they fit; handing them over as such
+        # spares its scan of the 64-bit arrays and its own converting copy
+        ir, jc = ir.astype(np.int32), jc.astype(np.int32)
     eqs = sp.csc_matrix((pr, ir, jc), shape=(nrow, 2 * K + 1))
     return clstack, corrstack, eqs, emap
 
