@@ -150,6 +150,7 @@ struct TriArgs {
     // SCORES
     int n_intervals;
     const double* thr;          // [n_intervals-1] thresholds (repeated additions of 1/n)
+    int thr_lds;                // 1: the thresholds are copied to LDS after the counts
     double* cum;                // [P] or null
     unsigned long long* hist;   // [n_intervals] counts
     // PROBS: P2, A, a, B, b, x0; and, when lform, log A, log(P2 B),
@@ -211,7 +212,7 @@ __device__ inline void f_both(double x, const double* prm, const double* lg, dou
 // staged 12 at a time (43 KB of LDS at NEC = 4: three workgroups per CU),
 // the other modes' 16 at a time.  The stage size does not change the order
 // of any sum.
-template <int MODE, int NEC = 1>
+template <int MODE, int NEC = 1, bool TL = false>
 __device__ __forceinline__ void tri_body(TriArgs a) {
     constexpr int KCm = MODE == SIGNS ? 12 : KC;
     __shared__ double Lp[9][KCm][TB];
@@ -221,11 +222,17 @@ __device__ __forceinline__ void tri_body(TriArgs a) {
     __shared__ double Sp[MODE == SIGNS ? KCm : 1][TB];
     __shared__ double Sq[MODE == SIGNS ? KCm : 1][TB];
     extern __shared__ unsigned int s_hist[];
+    // the thresholds after the counts, 8-byte aligned (the first stage's
+    // barrier orders these writes before any read)
+    double* s_thr = (double*)(s_hist + ((a.n_intervals + 1) & ~1));
     const int N = a.N;
     const long long NN = (long long)N * N;
     const int tid = threadIdx.x;
-    if (MODE == SCORES)
+    if (MODE == SCORES) {
         for (int l = tid; l < a.n_intervals; l += blockDim.x) s_hist[l] = 0;
+        if (TL)
+            for (int l = tid; l < a.n_intervals - 1; l += blockDim.x) s_thr[l] = a.thr[l];
+    }
     for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
         const int2 bt = a.tiles[t];
         const int p0 = bt.x * TB, q0 = bt.y * TB;
@@ -292,7 +299,8 @@ __device__ __forceinline__ void tri_body(TriArgs a) {
                 if constexpr (MODE == SCORES) {
                     acc0 += sa + sb;
                     const double hs = role == 0 ? sa : (role == 1 ? sb : sb);
-                    atomicAdd(&s_hist[bin_of(hs, a.thr, a.n_intervals)], 1u);
+                    const int bin = TL ? bin_of(hs, s_thr, a.n_intervals) : bin_of(hs, a.thr, a.n_intervals);
+                    atomicAdd(&s_hist[bin], 1u);
                 } else if constexpr (MODE == PROBS) {
                     if (a.lform) {
                         double ia, aa, ib, ab;
@@ -370,7 +378,8 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 // (four waves per SIMD); the probabilities (log, exp, log1p) need more and
 // run at two; the signs product runs at three for up to four vectors (its
 // LDS and registers sized for them), at two for up to eight.
-__global__ __launch_bounds__(256, 4) void tri_scores_kernel(TriArgs a) { tri_body<SCORES>(a); }
+template <bool TL>
+__global__ __launch_bounds__(256, 4) void tri_scores_kernel(TriArgs a) { tri_body<SCORES, 1, TL>(a); }
 __global__ __launch_bounds__(256) void tri_probs_kernel(TriArgs a) { tri_body<PROBS>(a); }
 __global__ __launch_bounds__(256, 3) void tri_signs4_kernel(TriArgs a) { tri_body<SIGNS, 4>(a); }
 __global__ __launch_bounds__(256) void tri_signs8_kernel(TriArgs a) { tri_body<SIGNS, MAXE>(a); }
@@ -436,7 +445,10 @@ template <int MODE>
 int s3_launch(clm_ctx* c, const s3::TriArgs& a, size_t dyn_lds) {
     const unsigned grid = (unsigned)std::min<long long>(a.ntiles, (long long)c->n_cu * 8);
     if constexpr (MODE == s3::SCORES)
-        s3::tri_scores_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
+        if (a.thr_lds)
+            s3::tri_scores_kernel<true><<<grid, 256, dyn_lds, c->stream>>>(a);
+        else
+            s3::tri_scores_kernel<false><<<grid, 256, dyn_lds, c->stream>>>(a);
     else if constexpr (MODE == s3::PROBS)
         s3::tri_probs_kernel<<<grid, 256, dyn_lds, c->stream>>>(a);
     else if (a.ne <= 4)
@@ -476,7 +488,11 @@ extern "C" int clm_sync3n_triangles_scores(clm_ctx* c, int32_t N, const double* 
     a.thr = (const double*)c->s3_thr.p;
     a.cum = d_cum;
     a.hist = (unsigned long long*)c->s3_hist.p;
-    if ((rc = s3_launch<SCORES>(c, a, (size_t)n_intervals * sizeof(unsigned int)))) return rc;
+    // thresholds in LDS for up to 1024 intervals (8 KB)
+    a.thr_lds = n_intervals <= 1024;
+    const size_t lds = (size_t)((n_intervals + 1) & ~1) * sizeof(unsigned int) +
+                       (a.thr_lds ? (size_t)(n_intervals - 1) * sizeof(double) : 0);
+    if ((rc = s3_launch<SCORES>(c, a, lds))) return rc;
     hist_out_kernel<<<(n_intervals + 255) / 256, 256, 0, c->stream>>>(n_intervals, a.hist, d_hist);
     HIPCHK(c, hipGetLastError());
     rc = s3_finish(c, a, st);

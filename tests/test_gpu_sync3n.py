@@ -43,7 +43,7 @@ def test_triangles_scores(oracle, N):
 def test_triangles_scores_bins_and_no_cum(oracle):
     say("case sync3n bins")
     R, _ = noisy(33, 7)
-    for n in (1, 7, 256):
+    for n in (1, 7, 256, 1024, 1025, 3000):  # thresholds in LDS up to 1024 intervals, global beyond
         c, h, _ = triangles_scores_mex(R, n, cum=False)
         assert c is None
         assert np.array_equal(h, oracle.triangles_scores(R, n, cum=False)[1]), n
