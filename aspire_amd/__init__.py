@@ -3,6 +3,7 @@
 from .clmatrix import (assemble, autotune, clmatrix_grouped, clmatrix_records, cryo_clmatrix,  # noqa: F401
                        cryo_clmatrix_subset, partner_subset)
 from .pft import cryo_pft  # noqa: F401
+from .shifts import cryo_estimate_shifts  # noqa: F401
 
 __all__ = ["cryo_clmatrix", "clmatrix_records", "clmatrix_grouped", "autotune", "assemble", "cryo_pft",
-           "cryo_clmatrix_subset", "partner_subset"]
+           "cryo_clmatrix_subset", "partner_subset", "cryo_estimate_shifts"]

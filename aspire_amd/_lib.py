@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libclm.so")
 CLM_FP32, CLM_BF16_RESCORE, CLM_FP16_RESCORE, CLM_FP64_REF = 0, 1, 2, 3
 CLM_C128, CLM_C64 = 0, 1
 CLM_R64, CLM_R32 = 0, 1
+CLM_FLIP_BY_INDEX = 1  # orientation of an equation's second line by its index
 CLM_PFT_EXACT, CLM_PFT_NUFFT = 0, 1
 
 PRECISIONS = {
@@ -36,7 +37,8 @@ EXPORTS = (
     "clm_set_log", "clm_clstack_mask",
     "clm_sync3n_triangles_scores", "clm_sync3n_pairs_probabilities", "clm_sync3n_signs_times_v",
     "clm_clmatrix_subset_device", "clm_clmatrix_subset", "clm_partner_subset", "clm_assemble_subset",
-    "clm_shift_equations_csc_subset",
+    "clm_shift_equations_csc_subset", "clm_shift_equations_csc_ex", "clm_estimate_shifts_ex",
+    "clm_common_lines_from_rotations",
 )
 
 # void (*)(void *user, int level, const char *msg)
@@ -282,6 +284,15 @@ def load():
     L.clm_shift_equations_csc_subset.restype = ctypes.c_int
     L.clm_shift_equations_csc_subset.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp,
                                                  ctypes.POINTER(ctypes.c_int64)]
+    L.clm_shift_equations_csc_ex.restype = ctypes.c_int
+    L.clm_shift_equations_csc_ex.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, vp, vp,
+                                             vp, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.clm_estimate_shifts_ex.restype = ctypes.c_int
+    L.clm_estimate_shifts_ex.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, ctypes.c_double,
+                                         ctypes.c_int32, ctypes.c_double, ctypes.c_int32, vp,
+                                         ctypes.POINTER(ClmLsqrStats)]
+    L.clm_common_lines_from_rotations.restype = ctypes.c_int
+    L.clm_common_lines_from_rotations.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp]
     L.clm_pft_freqs.restype = ctypes.c_int
     L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L

@@ -226,13 +226,50 @@ def autotune(pf, max_shift=15, shift_step=1.0, precision="fp16", device=0, sampl
     return rows, best
 
 
-def assemble(K: int, n_theta: int, rec, sel=None):
+def shift_equations(K: int, n_theta: int, cl1, cl2, shift, sel=None, flags=0):
+    """shift_equations (scipy CSC, rows x (2K+1), the right-hand side last)
+    from per-pair lines and 1-D shifts (host FP64, libclm's
+    clm_shift_equations_csc_ex): one row per pair in pair order, or per
+    selected pair with ``sel`` (P bytes); ``flags`` ``_lib.CLM_FLIP_BY_INDEX``
+    builds the rows as cryo_estimate_shifts does."""
+    import scipy.sparse as sp
+
+    L = _lib.load()
+    P = max(K * (K - 1) // 2, 0)
+    selh = None
+    if sel is not None:
+        selh = np.ascontiguousarray(sel, dtype=np.uint8)
+        if selh.shape != (P,):
+            raise ValueError("sel must hold K(K-1)/2 bytes")
+    cl1 = np.ascontiguousarray(cl1, dtype=np.int32)
+    cl2 = np.ascontiguousarray(cl2, dtype=np.int32)
+    shift = np.ascontiguousarray(shift, dtype=np.float64)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+    nrow = P if selh is None else int(np.count_nonzero(selh))
+    csc = lambda *a: L.clm_shift_equations_csc_ex(int(K), int(n_theta), vp(selh), int(flags), *a)
+    nnz = ctypes.c_int64(0)
+    jc = np.zeros(2 * K + 2, np.int64)
+    rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), None, None, ctypes.byref(nnz))
+    if rc:
+        raise _lib.ClmError(f"clm_shift_equations_csc_ex failed ({rc})")
+    ir = np.zeros(nnz.value, np.int64)
+    pr = np.zeros(nnz.value, np.float64)
+    rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), vp(ir), vp(pr), ctypes.byref(nnz))
+    if rc:
+        raise _lib.ClmError(f"clm_shift_equations_csc_ex failed ({rc})")
+    if nrow < 2 ** 31 and nnz.value < 2 ** 31:
+        # scipy keeps 32-bit indices when they fit; handing them over as such
+        # spares its scan of the 64-bit arrays and its own converting copy
+        ir, jc = ir.astype(np.int32), jc.astype(np.int32)
+    return sp.csc_matrix((pr, ir, jc), shape=(nrow, 2 * K + 1))
+
+
+def assemble(K: int, n_theta: int, rec, sel=None, flags=0):
     """Reference outputs from per-pair records (FP64, host; libclm's
     clm_assemble and clm_shift_equations_csc).  With ``sel`` (P bytes), the
     partner-subset form: equations for the selected pairs only, numbered in
-    pair order (clm_assemble_subset, clm_shift_equations_csc_subset)."""
-    import scipy.sparse as sp
-
+    pair order (clm_assemble_subset, clm_shift_equations_csc_ex).  ``flags``
+    (``_lib.CLM_FLIP_BY_INDEX``) builds the rows as cryo_estimate_shifts does."""
     L = _lib.load()
     if sel is not None:
         selh = np.ascontiguousarray(sel, dtype=np.uint8)
@@ -241,37 +278,18 @@ def assemble(K: int, n_theta: int, rec, sel=None):
     cl1 = np.ascontiguousarray(rec["cl1"], dtype=np.int32)
     cl2 = np.ascontiguousarray(rec["cl2"], dtype=np.int32)
     corr = np.ascontiguousarray(rec["corr"], dtype=np.float64)
-    shift = np.ascontiguousarray(rec["shift"], dtype=np.float64)
     clstack = np.zeros((K, K), order="F")
     corrstack = np.zeros((K, K), order="F")
     emap = np.zeros((K, K), order="F")
     vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     if sel is None:
         rc = L.clm_assemble(int(K), vp(cl1), vp(cl2), vp(corr), vp(clstack), vp(corrstack), vp(emap))
-        nrow = K * (K - 1) // 2
-        csc = lambda *a: L.clm_shift_equations_csc(int(K), int(n_theta), *a)
     else:
         rc = L.clm_assemble_subset(int(K), vp(selh), vp(cl1), vp(cl2), vp(corr), vp(clstack), vp(corrstack),
                                    vp(emap))
-        nrow = int(np.count_nonzero(selh))
-        csc = lambda *a: L.clm_shift_equations_csc_subset(int(K), int(n_theta), vp(selh), *a)
     if rc:
         raise _lib.ClmError(f"clm_assemble failed ({rc})")
-    nnz = ctypes.c_int64(0)
-    jc = np.zeros(2 * K + 2, np.int64)
-    rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), None, None, ctypes.byref(nnz))
-    if rc:
-        raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
-    ir = np.zeros(nnz.value, np.int64)
-    pr = np.zeros(nnz.value, np.float64)
-    rc = csc(vp(cl1), vp(cl2), vp(shift), vp(jc), vp(ir), vp(pr), ctypes.byref(nnz))
-    if rc:
-        raise _lib.ClmError(f"clm_shift_equations_csc failed ({rc})")
-    if nrow < 2 ** 31 and nnz.value < 2 ** 31:
-        # scipy keeps 32-bit indices when they fit; handing them over as such
-        # spares its scan of the 64-bit arrays and its own converting copy
-        ir, jc = ir.astype(np.int32), jc.astype(np.int32)
-    eqs = sp.csc_matrix((pr, ir, jc), shape=(nrow, 2 * K + 1))
+    eqs = shift_equations(K, n_theta, cl1, cl2, rec["shift"], sel, flags)
     return clstack, corrstack, eqs, emap
 
 

@@ -1573,8 +1573,9 @@ int assemble_impl(int32_t K, const uint8_t* sel, const int32_t* cl1, const int32
     return CLM_OK;
 }
 
-int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2,
-                         const double* shift1d, int64_t* jc, int64_t* ir, double* pr, int64_t* nnz);
+int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, int32_t flags, const int32_t* cl1,
+                         const int32_t* cl2, const double* shift1d, int64_t* jc, int64_t* ir, double* pr,
+                         int64_t* nnz);
 }  // namespace
 
 extern "C" {
@@ -1715,6 +1716,42 @@ int clm_clmatrix_subset_device(clm_ctx* c, const clm_params* p, const void* d_pf
     const int rc = run_subset(c, p, d_sel, out, st);
     if (rc == CLM_OK && st && st->pairs > 0) log_job(c, 1, p, st);
     return rc;
+}
+
+int clm_common_lines_from_rotations(clm_ctx* c, const clm_params* p, const void* d_pf, const double* d_rot,
+                                    const uint8_t* d_sel, int32_t* d_cl1, int32_t* d_cl2, double* d_shift1d) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    const long long P = clm_num_pairs(p->K);
+    if (P > 0 && (!d_pf || !d_rot || !d_cl1 || !d_cl2 || !d_shift1d)) return fail(c, CLM_E_ARG, "null pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    // the rays: the FP64 preprocessing of the search (lines, filter, band,
+    // normalization; cryo_estimate_shifts.m:63,121,180-188 is that of
+    // cryo_clmatrix), without its A-stacks or records
+    clm_params q = *p;
+    q.precision = CLM_FP64_REF;
+    q.variant = 0;
+    const clm_stack_io io{d_pf, d_cl1, d_cl2, nullptr, nullptr, d_shift1d};
+    Plan pl;
+    int rc;
+    if ((rc = prepare(c, pl, 1, &q, &io, false, nullptr, 0, false))) return rc;
+    if (P == 0) return CLM_OK;
+    if ((rc = enqueue_fill(c, c->ctr.p, 32, 0))) return rc;
+    if ((rc = enqueue_prep(c, pl))) return rc;
+    // a wave per 64 consecutive pairs, eight waves per workgroup; the shift
+    // phases in LDS when they fit in 64 KB (41 shifts x 64 ray points: 42 KB)
+    const Geom& g = pl.jobs[0].s.g;
+    const size_t ph_bytes = (size_t)g.S * g.R * sizeof(double2);
+    const unsigned grid = (unsigned)std::min<long long>((P + 511) / 512, (long long)c->n_cu * 8);
+    if (ph_bytes <= 65536)
+        hipLaunchKernelGGL(rotlines_kernel<true>, dim3(grid), dim3(512), ph_bytes, c->stream,
+                           (const StackDev*)c->table.p, d_rot, d_sel, (int)p->n_theta, d_cl1, d_cl2, d_shift1d);
+    else
+        hipLaunchKernelGGL(rotlines_kernel<false>, dim3(grid), dim3(512), 0, c->stream, (const StackDev*)c->table.p,
+                           d_rot, d_sel, (int)p->n_theta, d_cl1, d_cl2, d_shift1d);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
 }
 
 int clm_autotune(clm_ctx* c, const clm_params* p, const void* d_pf, int32_t sample_pairs, clm_tune_entry* entries,
@@ -1931,14 +1968,21 @@ int clm_partner_subset(int32_t K, int32_t NK, uint64_t seed, uint8_t* sel, int64
 
 int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, const int32_t* cl2,
                             const double* shift1d, int64_t* jc, int64_t* ir, double* pr, int64_t* nnz) {
-    return shift_equations_impl(K, n_theta, nullptr, cl1, cl2, shift1d, jc, ir, pr, nnz);
+    return shift_equations_impl(K, n_theta, nullptr, 0, cl1, cl2, shift1d, jc, ir, pr, nnz);
+}
+
+int clm_shift_equations_csc_ex(int32_t K, int32_t n_theta, const uint8_t* sel, int32_t flags, const int32_t* cl1,
+                               const int32_t* cl2, const double* shift1d, int64_t* jc, int64_t* ir, double* pr,
+                               int64_t* nnz) {
+    if (flags & ~CLM_FLIP_BY_INDEX) return CLM_E_ARG;
+    return shift_equations_impl(K, n_theta, sel, flags, cl1, cl2, shift1d, jc, ir, pr, nnz);
 }
 
 int clm_shift_equations_csc_subset(int32_t K, int32_t n_theta, const uint8_t* sel, const int32_t* cl1,
                                    const int32_t* cl2, const double* shift1d, int64_t* jc, int64_t* ir, double* pr,
                                    int64_t* nnz) {
     if (K > 1 && !sel) return CLM_E_ARG;
-    return shift_equations_impl(K, n_theta, sel, cl1, cl2, shift1d, jc, ir, pr, nnz);
+    return shift_equations_impl(K, n_theta, sel, 0, cl1, cl2, shift1d, jc, ir, pr, nnz);
 }
 
 }  // extern "C"
@@ -1946,8 +1990,9 @@ int clm_shift_equations_csc_subset(int32_t K, int32_t n_theta, const uint8_t* se
 namespace {
 // shift_equations in CSC form; with a selection, only the selected pairs'
 // rows, numbered in pair order (cryo_clmatrix_gpu_1.m:809-827,853-858)
-int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2,
-                         const double* shift1d, int64_t* jc, int64_t* ir, double* pr, int64_t* nnz) {
+int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, int32_t flags, const int32_t* cl1,
+                         const int32_t* cl2, const double* shift1d, int64_t* jc, int64_t* ir, double* pr,
+                         int64_t* nnz) {
     if (K < 0 || n_theta <= 0 || n_theta % 2 || !cl1 || !cl2 || !shift1d) return CLM_E_ARG;
     const int L = n_theta / 2;
     const double dtheta = M_PI / (double)L;  // :245
@@ -1959,7 +2004,9 @@ int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, const i
         sa[cl] = std::sin(a);
         ca[cl] = std::cos(a);
         double b = a;
-        lowb[cl] = b < M_PI;
+        // the second line keeps its orientation: angle test (cryo_clmatrix)
+        // or index test (cryo_estimate_shifts, CLM_FLIP_BY_INDEX)
+        lowb[cl] = (flags & CLM_FLIP_BY_INDEX) ? cl <= L : b < M_PI;
         if (!lowb[cl]) b = b - M_PI;
         sb[cl] = std::sin(b);
         cb[cl] = std::cos(b);

@@ -668,6 +668,142 @@ __global__ void window_merge_kernel(int K, int L, const double* __restrict__ lsh
     }
 }
 
+// Common lines of known rotations and the best 1-D shift along each: the
+// per-pair loop of cryo_estimate_shifts.m:142-223 (pair order k1 < k2; the
+// reference draws its pairs in random order, which the solution does not
+// depend on).  One wave per pair; lane s evaluates the shifts s, s + 64, ...
+// with the FP64 terms of eval64 added in order over the ray, then the wave
+// finds the first maximum of C1 and of C2 (NaN ignored, as MATLAB's max).
+__device__ inline bool first_max(double v, int s, double bv, int bs) {
+    return !isnan(v) && (isnan(bv) || v > bv || (v == bv && s < bs));
+}
+
+// Read-only FP64 data read through the constant address space: loads from a
+// wave-uniform address become scalar loads.
+typedef const double __attribute__((address_space(4)))* ConstD;
+
+// One wave takes 64 consecutive pairs at a time: each lane writes the
+// outputs of its pair if the selection leaves it out (coalesced), then the
+// wave computes the selected ones one after another (a ballot lists them).
+// With PH_LDS the shift phases are staged in LDS once per workgroup as
+// [r][s], so that the lanes (one per shift) read consecutive entries at
+// each ray index; otherwise they are read from the [s][r] table in global
+// memory.  Eight waves per workgroup share one staged table.
+
+template <bool PH_LDS>
+__global__ __launch_bounds__(512) void rotlines_kernel(const StackDev* __restrict__ sdp,
+                                                       const double* __restrict__ rot,
+                                                       const uint8_t* __restrict__ sel, int n_theta,
+                                                       int32_t* __restrict__ cl1, int32_t* __restrict__ cl2,
+                                                       double* __restrict__ shift1d) {
+    extern __shared__ double2 s_ph[];  // [R][S] when PH_LDS
+    const StackDev& sd = *sdp;
+    const Geom g = sd.g;
+    const long long K = g.K, P = K * (K - 1) / 2;
+    const int lane = threadIdx.x & 63;
+    if (PH_LDS) {
+        for (int i = threadIdx.x; i < g.S * g.R; i += blockDim.x) {
+            const int s = i / g.R, r = i - s * g.R;
+            s_ph[r * g.S + s] = sd.ph64[i];
+        }
+        __syncthreads();
+    }
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    // the wave's index, made visibly uniform (scalar) for the compiler
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (long long base = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * 64; base < P; base += nw * 64) {
+        const long long el = base + lane;
+        const bool act = el < P && (!sel || sel[el]);
+        if (el < P && !act) {
+            cl1[el] = -1;
+            cl2[el] = -1;
+            shift1d[el] = 0.0;
+        }
+        unsigned long long todo = __ballot(act);
+        while (todo) {
+            const long long e = base + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            // (k1, k2) of pair e
+            long long k1 = (long long)floor(((2.0 * K - 1.0) - sqrt((2.0 * K - 1.0) * (2.0 * K - 1.0) - 8.0 * (double)e)) / 2.0);
+            if (k1 < 0) k1 = 0;
+            if (k1 > K - 2) k1 = K - 2;
+            auto first = [&](long long x) { return x * K - x * (x + 1) / 2; };
+            while (k1 > 0 && first(k1) > e) k1--;
+            while (k1 + 1 <= K - 2 && first(k1 + 1) <= e) k1++;
+            const long long k2 = e - first(k1) + k1 + 1;
+            // commonline_R(R_k1.', R_k2.', n_theta): Ut = R_k2.' * R_k1, entry
+            // (a, b) = sum over m of R_k2(m, a) R_k1(m, b), m in order
+            const double* Ri = rot + 9 * k1;
+            const double* Rj = rot + 9 * k2;
+            auto ut = [&](int a, int b) {
+                return (Rj[3 * a] * Ri[3 * b] + Rj[1 + 3 * a] * Ri[1 + 3 * b]) + Rj[2 + 3 * a] * Ri[2 + 3 * b];
+            };
+            const double PI = M_PI;  // 4*atan(1.0) in commonline_R.m:27
+            const double aij = atan2(ut(2, 0), -ut(2, 1)) + PI;
+            const double aji = atan2(-ut(0, 2), ut(1, 2)) + PI;
+            // round half away from zero, then mod n_theta; angles are in [0, 2 pi]
+            // (a rotation with NaN entries gives line 0)
+            double xij = round(aij / (2 * PI) * n_theta), xji = round(aji / (2 * PI) * n_theta);
+            if (!(xij >= 0.0 && xij <= n_theta)) xij = 0.0;
+            if (!(xji >= 0.0 && xji <= n_theta)) xji = 0.0;
+            int cij = (int)xij % n_theta;
+            int cji = (int)xji % n_theta;
+            const int L2 = n_theta / 2;
+            if (cij >= L2) {  // cryo_estimate_shifts.m:155-163
+                cij -= L2;
+                cji -= L2;
+            }
+            if (cji < 0) cji += n_theta;
+            cij += 1;
+            cji += 1;
+            // the two rays are the same for every lane: read through the
+            // constant address space, so that they come in by scalar loads
+            // (one per wave) instead of a vector load of one address per lane
+            // (values computed in vector registers from uniform inputs; read
+            // from the first lane so that the addresses are scalar)
+            const int lj = __builtin_amdgcn_readfirstlane(cji <= L2 ? cji - 1 : cji - 1 - L2);
+            const int li = __builtin_amdgcn_readfirstlane(cij - 1);
+            const int q1 = __builtin_amdgcn_readfirstlane((int)k1), q2 = __builtin_amdgcn_readfirstlane((int)k2);
+            const ConstD ri = (ConstD)(sd.P64 + ((size_t)q1 * g.L + li) * g.R);
+            const ConstD rj = (ConstD)(sd.P64 + ((size_t)q2 * g.L + lj) * g.R);
+            double b1 = NAN, b2 = NAN;
+            int s1 = 0, s2 = 0;
+            // the shift loop is uniform (lanes past the last shift compute
+            // the last one again and are ignored), so the ray loop runs
+            // outside divergent control and its ray addresses stay scalar
+            for (int s0 = 0; s0 < g.S; s0 += 64) {
+                const int s = s0 + lane;
+                const int sc = s < g.S ? s : g.S - 1;
+                double a1 = 0.0, a2 = 0.0;
+#pragma unroll 4
+                for (int r = 0; r < g.R; r++) {
+                    const double2 pv = make_double2(ri[2 * r], ri[2 * r + 1]);
+                    const double2 bv = make_double2(rj[2 * r], rj[2 * r + 1]);
+                    const double2 qv = PH_LDS ? s_ph[r * g.S + sc] : sd.ph64[(size_t)sc * g.R + r];
+                    a1 = a1 + term64(pv, qv, bv, false);
+                    a2 = a2 + term64(pv, qv, bv, true);
+                }
+                const double c1 = 2.0 * a1, c2 = 2.0 * a2;
+                if (s < g.S && first_max(c1, s, b1, s1)) { b1 = c1; s1 = s; }
+                if (s < g.S && first_max(c2, s, b2, s2)) { b2 = c2; s2 = s; }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v1 = __shfl_xor(b1, o, 64), v2 = __shfl_xor(b2, o, 64);
+                const int t1 = __shfl_xor(s1, o, 64), t2 = __shfl_xor(s2, o, 64);
+                if (first_max(v1, t1, b1, s1)) { b1 = v1; s1 = t1; }
+                if (first_max(v2, t2, b2, s2)) { b2 = v2; s2 = t2; }
+            }
+            if (lane == 0) {
+                // an all-NaN max is NaN at index 1 (s = 0) in MATLAB, as here
+                cl1[e] = cij;
+                cl2[e] = cji;
+                shift1d[e] = sd.shiftv[b1 > b2 ? s1 : s2];
+            }
+        }
+    }
+}
+
 // Partner-subset search: the pairs left out keep the outputs of a pair with
 // no common line (cryo_clmatrix_gpu_1.m:346-350 searches only the drawn
 // partners; clstack and corrstack stay zero elsewhere).  The selected pairs'

@@ -52,7 +52,7 @@ __device__ inline long long first(long long k1, long long K) { return k1 * K - k
 // line numbers c = 0..2L (0 = no line; cryo_clmatrix_cpu.m:722-741).  Entry
 // 2L+1 is all zeros: the row of a pair whose line numbers lie outside
 // [0, 2L].
-__global__ void tables_kernel(int L, double2* ta, double* tb) {
+__global__ void tables_kernel(int L, int by_index, double2* ta, double* tb) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > 2 * L + 1) return;
     if (c == 2 * L + 1) {
@@ -65,7 +65,11 @@ __global__ void tables_kernel(int L, double2* ta, double* tb) {
     double b = (double)(c - 1) * dtheta;
     ta[c] = make_double2(sin(a), cos(a));
     double sg = 1.0;
-    if (!(b < M_PI)) {
+    // the second line's orientation: the angle test of cryo_clmatrix_cpu.m:733
+    // (beta < pi), or with by_index the line index of
+    // cryo_estimate_shifts.m:171-176,215-222 (c > L); they differ only at c = L + 1
+    // for the L where L * RN(pi/L) < RN(pi)
+    if (by_index ? c > L : !(b < M_PI)) {
         b = b - M_PI;
         sg = -1.0;
     }
@@ -545,11 +549,12 @@ __global__ __launch_bounds__(256) void finish_kernel(int n2, const State* S, con
 
 using namespace clmh;
 
-extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const int32_t* d_cl1,
-                                   const int32_t* d_cl2, const double* d_shift1d, double tol, int32_t maxit,
-                                   double damp, double* d_x, clm_lsqr_stats* st) {
+extern "C" int clm_estimate_shifts_ex(clm_ctx* c, int32_t K, int32_t n_theta, const int32_t* d_cl1,
+                                      const int32_t* d_cl2, const double* d_shift1d, double tol, int32_t maxit,
+                                      double damp, int32_t flags, double* d_x, clm_lsqr_stats* st) {
     using namespace lsq;
     if (!c) return CLM_E_ARG;
+    if (flags & ~CLM_FLIP_BY_INDEX) return fail(c, CLM_E_ARG, "unknown flags 0x%x", flags);
     if (st) *st = clm_lsqr_stats{};
     if (K < 2) return fail(c, CLM_E_ARG, "K = %d: at least two images are needed", K);
     if (K > 65536) return fail(c, CLM_E_UNSUP, "K = %d exceeds 65536", K);
@@ -599,7 +604,7 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
     h.flag = maxit == 0 ? 1 : 0;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, hipMemcpyAsync(S, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
-    tables_kernel<<<(2 * L + 2 + 255) / 256, 256, 0, c->stream>>>(L, ta, tb);
+    tables_kernel<<<(2 * L + 2 + 255) / 256, 256, 0, c->stream>>>(L, (flags & CLM_FLIP_BY_INDEX) ? 1 : 0, ta, tb);
     // coefficient tables in LDS when they are small (n_theta up to 816)
     const size_t tab_bytes = (size_t)5 * (2 * L + 2) * 8;
     const bool lt = tab_bytes <= 32768;
@@ -659,4 +664,10 @@ extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const
         st->ms = ms;
     }
     return CLM_OK;
+}
+
+extern "C" int clm_estimate_shifts(clm_ctx* c, int32_t K, int32_t n_theta, const int32_t* d_cl1,
+                                   const int32_t* d_cl2, const double* d_shift1d, double tol, int32_t maxit,
+                                   double damp, double* d_x, clm_lsqr_stats* st) {
+    return clm_estimate_shifts_ex(c, K, n_theta, d_cl1, d_cl2, d_shift1d, tol, maxit, damp, 0, d_x, st);
 }

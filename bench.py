@@ -649,6 +649,7 @@ def shifts_bench(args):
     st0 = sim.make_stack(K=K, n=129, n_r=65, n_theta=n_theta, snr=1 / 16, max_shift_2d=5.0, shift_kind="uniform",
                          seed=1003, device="cuda")
     rec, _ = clmatrix_records(st0.pf, 10, 0.5, precision="bf16")
+    drop_in = estimate_shifts_dropin(st0, K, n_theta)
     del st0
     torch.cuda.synchronize()
     P = K * (K - 1) // 2
@@ -697,8 +698,55 @@ def shifts_bench(args):
                             "sample": f"{n_cpu} LSQR iterations of the oracle (scipy CSR) on the full system, "
                                       f"scaled to {it} iterations"},
            "parity_sample": {"iterations": n_cpu,
-                             "max_abs_diff_rel_to_max": float(np.abs(xg - xo).max() / np.abs(xo).max())}}
+                             "max_abs_diff_rel_to_max": float(np.abs(xg - xo).max() / np.abs(xo).max())},
+           "cryo_estimate_shifts": drop_in}
     print(json.dumps(out))
+
+
+def estimate_shifts_dropin(st0, K, n_theta, max_shift=10, shift_step=0.5):
+    """The whole cryo_estimate_shifts(pf, rotations, 10, 0.5) on the stack's
+    true rotations (in the estimated rotations' convention), at the default
+    memoryfactor (10000 MB: 125,000 of the 12.5 M pairs at K=5000), and the
+    common-lines-from-rotations kernel over every pair; exact agreement of the
+    kernel's lines and 1-D shifts with the oracle on a sample of pairs."""
+    import torch
+
+    from aspire_amd.shifts import common_lines_from_rotations, cryo_estimate_shifts, n_equations
+    from oracle import clm_oracle, shifts_oracle
+
+    rot = np.ascontiguousarray(st0.rots.transpose(1, 0, 2))
+    cryo_estimate_shifts(st0.pf, rot, max_shift, shift_step)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    est, eqs = cryo_estimate_shifts(st0.pf, rot, max_shift, shift_step)
+    sec = time.perf_counter() - t
+    P = K * (K - 1) // 2
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    cl1, cl2, sh = common_lines_from_rotations(st0.pf, rot, max_shift, shift_step)
+    torch.cuda.synchronize()
+    lines_sec = time.perf_counter() - t
+    # parity sample: 64 seeded pairs against the oracle
+    sample = np.sort(np.random.default_rng(7).choice(P, size=64, replace=False))
+    k1, k2 = np.triu_indices(K, 1)
+    n_img = np.unique(np.concatenate([k1[sample], k2[sample]]))
+    sub = st0.pf[:, :, n_img].cpu().numpy() if isinstance(st0.pf, torch.Tensor) else st0.pf[:, :, n_img]
+    Ps, _ = clm_oracle.preprocess(sub)
+    pos = {int(k): i for i, k in enumerate(n_img)}
+    a = np.array([pos[int(k)] for k in k1[sample]])
+    b = np.array([pos[int(k)] for k in k2[sample]])
+    ora = shifts_oracle.rotation_pairs(Ps, rot[:, :, n_img], n_theta, max_shift, shift_step, a, b)
+    c1, c2, s1 = cl1.cpu().numpy()[sample], cl2.cpu().numpy()[sample], sh.cpu().numpy()[sample]
+    same = int(np.sum((c1 == ora["cl1"]) & (c2 == ora["cl2"]) & (s1 == ora["shift"])))
+    truth = st0.shifts.reshape(-1)
+    A = eqs[:, :-1]
+    resid = np.asarray(A @ truth - eqs[:, -1].toarray().ravel())
+    return {"equations": int(eqs.shape[0]), "n_equations": n_equations(K, 10000), "seconds": sec,
+            "max_shift": max_shift, "shift_step": shift_step,
+            "lines_all_pairs": {"pairs": P, "seconds": lines_sec, "pairs_per_s": P / lines_sec},
+            "parity_sample": {"pairs": len(sample), "identical_to_oracle": same},
+            "rms_residual_of_true_shifts": float(np.sqrt(np.mean(resid ** 2))),
+            "rms_rhs": float(np.sqrt(np.mean(eqs[:, -1].toarray() ** 2)))}
 
 
 def vote_bench(args):

@@ -399,6 +399,28 @@ int clm_shift_equations_csc_subset(int32_t K, int32_t n_theta,
                                    int64_t *nnz);
 
 /*
+ * Orientation rule of an equation's second line.  0: the angle test of
+ * cryo_clmatrix_cpu.m:733 (beta = (cl2-1)*pi/L < pi keeps the row,
+ * otherwise beta -= pi and the first two coefficients change sign).
+ * CLM_FLIP_BY_INDEX: the index test of cryo_estimate_shifts.m:171-176,
+ * 215-222 (cl2 > n_theta/2 flips).  The two differ only for cl2 =
+ * n_theta/2 + 1 at the n_theta where (n_theta/2) * RN(pi/(n_theta/2)) <
+ * RN(pi) (e.g. 150, 300, 350; not 360).
+ */
+#define CLM_FLIP_BY_INDEX 1
+
+/*
+ * clm_shift_equations_csc with an optional selection (sel null: every
+ * pair; else the rows of the selected pairs, as the _subset form) and the
+ * orientation rule of the second line (flags: 0 or CLM_FLIP_BY_INDEX).
+ */
+int clm_shift_equations_csc_ex(int32_t K, int32_t n_theta, const uint8_t *sel,
+                               int32_t flags, const int32_t *cl1,
+                               const int32_t *cl2, const double *shift1d,
+                               int64_t *jc, int64_t *ir, double *pr,
+                               int64_t *nnz);
+
+/*
  * Polar Fourier transform of a projection stack on the device: the step
  * before cryo_clmatrix.  Replaces [pf, freqs] = cryo_pft(p, n_r, n_theta)
  * (fourier/nufft/cryo_pft.m:1-75) with the exact non-uniform DFT that
@@ -496,6 +518,40 @@ int clm_estimate_shifts(clm_ctx *ctx, int32_t K, int32_t n_theta,
                         const int32_t *d_cl1, const int32_t *d_cl2,
                         const double *d_shift1d, double tol, int32_t maxit,
                         double damp, double *d_x, clm_lsqr_stats *stats);
+
+/* clm_estimate_shifts with an orientation rule (flags above). */
+int clm_estimate_shifts_ex(clm_ctx *ctx, int32_t K, int32_t n_theta,
+                           const int32_t *d_cl1, const int32_t *d_cl2,
+                           const double *d_shift1d, double tol, int32_t maxit,
+                           double damp, int32_t flags, double *d_x,
+                           clm_lsqr_stats *stats);
+
+/*
+ * Common lines and 1-D shifts from known rotations: the per-pair loop of
+ * cryo_estimate_shifts(pf, rotations, max_shift, shift_step, ...)
+ * (sinograms/cryo_estimate_shifts.m:142-223), the equations' lines and
+ * right-hand sides for clm_estimate_shifts_ex with CLM_FLIP_BY_INDEX.
+ * p gives K, n_r, n_theta, pf_dtype, max_shift and shift_step (the other
+ * fields are not used).  d_rot: K rotations, 3 x 3 x K doubles in MATLAB
+ * order (R_k(r,c) at r + 3c + 9k).  For every pair k1 < k2 with
+ * d_sel[e] != 0 (d_sel null: every pair), in equation order:
+ *   [cij, cji] = commonline_R(R_k1.', R_k2.', n_theta) (:151,
+ *     commonline_R.m:22-35, round half away from zero), moved to
+ *     cij < n_theta/2 (:155-163),
+ *     1-based: d_cl1[e] = cij in [1, n_theta/2], d_cl2[e] = cji in
+ *     [1, n_theta];
+ *   the rays of those lines, filtered and normalized as cryo_clmatrix does
+ *   (:166-188), and C1, C2 = 2 real((ray_i .* phases)' ray_j), 2 real((conj(
+ *   ray_i) .* phases)' ray_j) over the shift grid (:190-195; FP64, sums over
+ *   the ray in order); d_shift1d[e] = the shift of the first maximum of C1
+ *   if max C1 > max C2, else of C2 (:197-204).
+ * Pairs not selected get d_cl1 = d_cl2 = -1 (a zero row for the solver)
+ * and d_shift1d = 0.
+ */
+int clm_common_lines_from_rotations(clm_ctx *ctx, const clm_params *p,
+                                    const void *d_pf, const double *d_rot,
+                                    const uint8_t *d_sel, int32_t *d_cl1,
+                                    int32_t *d_cl2, double *d_shift1d);
 
 /*
  * The voting synchronization matrix on the device: the consumer of the

@@ -39,6 +39,8 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.clmo_n_shifts.restype = ctypes.c_int
         L.clmo_n_shifts.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.clmo_shift_value.restype = ctypes.c_double
+        L.clmo_shift_value.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.clmo_preprocess.restype = ctypes.c_int64
         L.clmo_preprocess.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
         L.clmo_pairs.restype = None

@@ -1,0 +1,71 @@
+"""Host parts of cryo_estimate_shifts (no GPU):
+
+* the shift equations as cryo_estimate_shifts.m:206-222 writes them (the
+  second line flipped by its index, libclm ``clm_shift_equations_csc_ex``
+  with ``CLM_FLIP_BY_INDEX``) equal the oracle's, for all pairs and for a
+  subset of them, at n_theta = 150 where the index rule and cryo_clmatrix's
+  angle rule part;
+* the number of equations (cryo_estimate_shifts.m:58-94) on hand-computed
+  cases, and its argument check.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from aspire_amd import _lib
+from aspire_amd.clmatrix import shift_equations
+from aspire_amd.shifts import n_equations
+from oracle import clm_oracle as co
+from oracle import shifts_oracle as so
+
+
+@pytest.mark.parametrize("subset", [False, True])
+def test_equations_by_index_match_oracle(subset):
+    K, n_theta = 40, 150
+    L = n_theta // 2
+    rng = np.random.default_rng(11)
+    P = K * (K - 1) // 2
+    cl1 = rng.integers(1, L + 1, P).astype(np.int32)
+    cl2 = rng.integers(1, 2 * L + 1, P).astype(np.int32)
+    cl2[::7] = L + 1  # the index rule flips these, the angle rule does not (L fl(pi/L) < pi)
+    cl2[3::7] = L
+    shift = rng.integers(-4, 5, P).astype(np.float64)
+    k1, k2 = co.all_pairs(K)
+    sel = None
+    if subset:
+        sel = (rng.uniform(size=P) < 0.3).astype(np.uint8)
+        keep = sel.astype(bool)
+    else:
+        keep = np.ones(P, bool)
+    eqs = shift_equations(K, n_theta, cl1, cl2, shift, sel, _lib.CLM_FLIP_BY_INDEX)
+    A, b = so.system(K, n_theta, cl1[keep], cl2[keep], shift[keep], flip_by_index=True, k1=k1[keep], k2=k2[keep])
+    ref = sp.hstack([A, sp.csr_matrix(b.reshape(-1, 1))], format="csc")
+    ref.eliminate_zeros()
+    assert eqs.shape == ref.shape == (int(keep.sum()), 2 * K + 1)
+    assert np.array_equal(eqs.indptr, ref.indptr) and np.array_equal(eqs.indices, ref.indices)
+    np.testing.assert_allclose(eqs.data, ref.data, rtol=0, atol=4e-16)
+    # the angle rule (flags 0) reverses exactly the rows with cl2 = L + 1
+    e0 = shift_equations(K, n_theta, cl1, cl2, shift, sel, 0).toarray()[:, :-1]
+    e1 = eqs.toarray()[:, :-1]
+    rev = cl2[keep] == L + 1
+    assert rev.any()
+    np.testing.assert_allclose(e0[rev], -e1[rev], rtol=0, atol=4e-16)
+    assert np.array_equal(e0[~rev], e1[~rev])
+
+
+def test_n_equations():
+    # memoryfactor <= 1: the fraction of the K(K-1)/2 pairs, at least K
+    assert n_equations(10, 0.3) == 14  # ceil(90 * 0.3 / 2) = ceil(13.5)
+    assert n_equations(10, 1.0) == 45
+    assert n_equations(10, 0.01) == 10  # ceil(0.45) = 1, raised to K
+    assert n_equations(10, 0) == 10
+    # memoryfactor in MB of the dense 45 x 20 system (7200 bytes): all pairs fit
+    assert n_equations(10, 100) == 45
+    # K = 1000: 499500 pairs, 7.992e9 bytes dense; 1000 MB is 62500 pairs in
+    # exact arithmetic, but the reference's doubles (fl(1e9/7.992e9) =
+    # 0.12512512512512514, times 999000, halved: 62500.00000000001) round up
+    assert n_equations(1000, 1000) == 62501
+    assert n_equations(1000, 500) == 31251  # 31250.000000000004 likewise
+    for bad in (-0.5, 1.5, 99):
+        with pytest.raises(ValueError):
+            n_equations(10, bad)
