@@ -39,6 +39,9 @@ EXPORTS = (
     "clm_clmatrix_subset_device", "clm_clmatrix_subset", "clm_partner_subset", "clm_assemble_subset",
     "clm_shift_equations_csc_subset", "clm_shift_equations_csc_ex", "clm_estimate_shifts_ex",
     "clm_common_lines_from_rotations",
+    "clm_cryo_estimate_shifts",
+    "clm_n_equations",
+    "clm_equation_pairs",
 )
 
 # void (*)(void *user, int level, const char *msg)
@@ -293,6 +296,14 @@ def load():
                                          ctypes.POINTER(ClmLsqrStats)]
     L.clm_common_lines_from_rotations.restype = ctypes.c_int
     L.clm_common_lines_from_rotations.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, vp, vp, vp]
+    L.clm_cryo_estimate_shifts.restype = ctypes.c_int
+    L.clm_cryo_estimate_shifts.argtypes = [vp, ctypes.POINTER(ClmParams), vp, vp, vp, ctypes.c_double, ctypes.c_int32,
+                                           vp, vp, vp, vp, ctypes.POINTER(ClmLsqrStats)]
+    L.clm_n_equations.restype = ctypes.c_int
+    L.clm_n_equations.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]
+    L.clm_equation_pairs.restype = ctypes.c_int
+    L.clm_equation_pairs.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, vp,
+                                     ctypes.POINTER(ctypes.c_int64)]
     L.clm_pft_freqs.restype = ctypes.c_int
     L.clm_pft_freqs.argtypes = [ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L

@@ -1654,6 +1654,7 @@ void clm_destroy(clm_ctx* c) {
     DevBuf* pb[] = {&c->pft_ax, &c->pft_ay, &c->pft_xt, &c->pft_img, &c->pft_out,
                     &c->pft_grid, &c->pft_base, &c->pft_wt, &c->pft_dec,
                     &c->ls_pack, &c->ls_u, &c->ls_part, &c->ls_vec, &c->ls_tab, &c->ls_state, &c->ls_tiles,
+                    &c->es_rot, &c->es_x,
                     &c->vt_m, &c->vt_mt, &c->vt_pk, &c->vt_cs,
                     &c->s3_rs, &c->s3_vs, &c->s3_ss, &c->s3_tiles, &c->s3_thr, &c->s3_hist};
     for (DevBuf* b : pb) release(*b);
@@ -1750,6 +1751,43 @@ int clm_common_lines_from_rotations(clm_ctx* c, const clm_params* p, const void*
         hipLaunchKernelGGL(rotlines_kernel<false>, dim3(grid), dim3(512), 0, c->stream, (const StackDev*)c->table.p,
                            d_rot, d_sel, (int)p->n_theta, d_cl1, d_cl2, d_shift1d);
     HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int clm_cryo_estimate_shifts(clm_ctx* c, const clm_params* p, const void* pf, const double* rot, const uint8_t* sel,
+                             double tol, int32_t maxit, int32_t* cl1, int32_t* cl2, double* shift1d, double* x,
+                             clm_lsqr_stats* st) {
+    if (!c) return CLM_E_ARG;
+    if (!p) return fail(c, CLM_E_ARG, "null params");
+    if (p->n_theta <= 0 || p->n_theta % 2) return fail(c, CLM_E_ARG, "n_theta must be even");
+    if (p->K < 2 || p->n_r < 2) return fail(c, CLM_E_ARG, "at least two images of two radial points are needed");
+    if (!pf || !rot || !cl1 || !cl2 || !shift1d || !x) return fail(c, CLM_E_ARG, "null pf, rotations or output");
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long P = clm_num_pairs(p->K);
+    const size_t esz = p->pf_dtype == CLM_C64 ? 8 : 16;
+    const size_t pf_bytes = (size_t)p->n_r * p->n_theta * p->K * esz;
+    int rc;
+    if ((rc = ensure(c, c->pf, pf_bytes)) || (rc = ensure(c, c->es_rot, (size_t)p->K * 9 * sizeof(double))) ||
+        (rc = ensure(c, c->es_x, (size_t)p->K * 2 * sizeof(double))) || (rc = ensure(c, c->out_cl1, P * 4)) ||
+        (rc = ensure(c, c->out_cl2, P * 4)) || (rc = ensure(c, c->out_shift, P * 8)) ||
+        (sel && (rc = ensure(c, c->w_flag, (size_t)P))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pf.p, pf, pf_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->es_rot.p, rot, (size_t)p->K * 9 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (sel) HIPCHK(c, hipMemcpyAsync(c->w_flag.p, sel, (size_t)P, hipMemcpyHostToDevice, c->stream));
+    if ((rc = clm_common_lines_from_rotations(c, p, c->pf.p, (const double*)c->es_rot.p,
+                                              sel ? (const uint8_t*)c->w_flag.p : nullptr, (int32_t*)c->out_cl1.p,
+                                              (int32_t*)c->out_cl2.p, (double*)c->out_shift.p)))
+        return rc;
+    if ((rc = clm_estimate_shifts_ex(c, p->K, p->n_theta, (const int32_t*)c->out_cl1.p,
+                                     (const int32_t*)c->out_cl2.p, (const double*)c->out_shift.p, tol, maxit, 0.0,
+                                     CLM_FLIP_BY_INDEX, (double*)c->es_x.p, st)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(cl1, c->out_cl1.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cl2, c->out_cl2.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(shift1d, c->out_shift.p, P * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(x, c->es_x.p, (size_t)p->K * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CLM_OK;
 }
@@ -1918,6 +1956,45 @@ int clm_clstack_mask(int32_t K, int32_t n_theta, const double* clstack, const do
 int clm_assemble(int32_t K, const int32_t* cl1, const int32_t* cl2, const double* corr, double* clstack,
                  double* corrstack, double* eqmap) {
     return assemble_impl(K, nullptr, cl1, cl2, corr, clstack, corrstack, eqmap);
+}
+
+int clm_n_equations(int32_t K, double memoryfactor, int64_t* n) {
+    // cryo_estimate_shifts.m:58-94, in the reference's double arithmetic
+    if (K < 0 || !n) return CLM_E_ARG;
+    if (!(memoryfactor >= 0.0) || (memoryfactor > 1.0 && memoryfactor < 100.0)) return CLM_E_ARG;
+    const double k = (double)K;
+    const double total = std::ceil(k * (k - 1.0) / 2.0);
+    const double memtotal = total * 2.0 * k * 8.0;
+    double neq;
+    if (memoryfactor <= 1.0) {
+        neq = std::ceil(k * (k - 1.0) * memoryfactor / 2.0);
+    } else {
+        const double f = (memoryfactor * 1e6) / memtotal;
+        neq = f < 1.0 ? std::ceil(k * (k - 1.0) * f / 2.0) : total;
+    }
+    *n = (int64_t)std::max(neq, k);
+    return CLM_OK;
+}
+
+int clm_equation_pairs(int32_t K, int64_t n, uint64_t seed, uint8_t* sel, int64_t* count) {
+    // n of the K(K-1)/2 pairs, every n-subset equally likely (the pairs
+    // cryo_estimate_shifts.m:130-137 draws with randperm; MATLAB's stream is
+    // not reproduced): selection sampling (Knuth, TAOCP vol. 2, 3.4.2,
+    // Algorithm S) over the pairs in order, one splitmix64 draw per pair
+    if (K < 0 || n < 0) return CLM_E_ARG;
+    const long long P = clm_num_pairs(K);
+    if (P > 0 && !sel) return CLM_E_ARG;
+    const long long want = std::min<long long>(n, P);
+    uint64_t st = seed ^ 0x9e3779b97f4a7c15ull;
+    long long got = 0;
+    for (long long i = 0; i < P; i++) {
+        const double u = (double)(splitmix64(st) >> 11) * 0x1p-53;  // [0, 1)
+        const bool take = (double)(P - i) * u < (double)(want - got);
+        sel[i] = take ? 1 : 0;
+        got += take;
+    }
+    if (count) *count = got;
+    return CLM_OK;
 }
 
 int clm_assemble_subset(int32_t K, const uint8_t* sel, const int32_t* cl1, const int32_t* cl2, const double* corr,

@@ -64,6 +64,7 @@ struct clm_ctx {
     // shift least squares: packed lines, u, per-tile sums, 2K vectors and the tiles' partial
     // A'u sums, tables, scalar state, pair tiles
     clmh::DevBuf ls_pack, ls_u, ls_part, ls_vec, ls_tab, ls_state, ls_tiles;
+    clmh::DevBuf es_rot, es_x;  // host-buffer form of cryo_estimate_shifts: rotations, solution
     int ls_tiles_K = 0, ls_ntiles = 0;  // the pair tiles held in ls_tiles are for this K
     // voting: clstack as 16-bit tables, row-major and transposed; per-pair histogram
     // peaks; cos and sin of the line-difference angles

@@ -16,7 +16,6 @@
 from __future__ import annotations
 
 import ctypes
-import math
 
 import numpy as np
 
@@ -106,20 +105,28 @@ def common_lines_from_rotations(pf, rotations, max_shift, shift_step=1.0, sel=No
 
 
 def n_equations(n_projs: int, memoryfactor: float) -> int:
-    """Number of pairs cryo_estimate_shifts uses (cryo_estimate_shifts.m:58-94):
-    all K(K-1)/2, a fraction memoryfactor of them (<= 1), or as many as fit in
-    memoryfactor megabytes of a dense K(K-1)/2 x 2K system (>= 100); at least K."""
-    if memoryfactor < 0 or 1 < memoryfactor < 100:
+    """Number of pairs cryo_estimate_shifts uses (cryo_estimate_shifts.m:58-94,
+    libclm ``clm_n_equations``): all K(K-1)/2, a fraction memoryfactor of them
+    (<= 1), or as many as fit in memoryfactor megabytes of a dense
+    K(K-1)/2 x 2K system (>= 100); at least K."""
+    n = ctypes.c_int64(0)
+    if _lib.load().clm_n_equations(int(n_projs), float(memoryfactor), ctypes.byref(n)):
         raise ValueError("subsamplingfactor must be between 0 and 1 or larger than 100")
-    n = int(n_projs)
-    total = math.ceil(n * (n - 1) / 2)
-    memtotal = total * 2 * n * 8
-    if memoryfactor <= 1:
-        neq = math.ceil(n * (n - 1) * memoryfactor / 2)
-    else:
-        f = memoryfactor * 10 ** 6 / memtotal
-        neq = math.ceil(n * (n - 1) * f / 2) if f < 1 else total
-    return max(neq, n)
+    return int(n.value)
+
+
+def equation_pairs(n_projs: int, n: int, seed: int = 0) -> np.ndarray:
+    """Selection (K(K-1)/2 uint8, 1 = drawn) of min(n, K(K-1)/2) pairs drawn
+    uniformly without replacement (libclm ``clm_equation_pairs``; the
+    reference draws them with randperm, cryo_estimate_shifts.m:130-137)."""
+    P = max(int(n_projs) * (int(n_projs) - 1) // 2, 0)
+    sel = np.zeros(max(P, 1), np.uint8)
+    cnt = ctypes.c_int64(0)
+    rc = _lib.load().clm_equation_pairs(int(n_projs), int(n), ctypes.c_uint64(seed & (2 ** 64 - 1)),
+                                        sel.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cnt))
+    if rc:
+        raise ValueError("invalid pair count")
+    return sel[:P]
 
 
 def cryo_estimate_shifts(pf, rotations, max_shift, shift_step=1.0, memoryfactor=10000, shifts_2d_ref=None,
@@ -130,7 +137,8 @@ def cryo_estimate_shifts(pf, rotations, max_shift, shift_step=1.0, memoryfactor=
     equations used (scipy CSC, rows x (2K+1), the right-hand side last).
 
     The reference picks n_equations(K, memoryfactor) pairs with ``randperm``
-    (:136-137); here they are a seeded uniform sample, and the equations come
+    (:136-137); here they are a seeded uniform sample (``equation_pairs``,
+    the same draw as the MEX gateway's for the same seed), and the equations come
     in pair order (the reference's rows are in its random order; the least
     squares does not depend on it).  Per pair, the common line of the
     rotations and the 1-D shift along it (:142-223) run on the GPU
@@ -144,11 +152,7 @@ def cryo_estimate_shifts(pf, rotations, max_shift, shift_step=1.0, memoryfactor=
     n_theta, K = shp[1], (shp[2] if len(shp) == 3 else 1)
     P = K * (K - 1) // 2
     neq = min(n_equations(K, memoryfactor), P)
-    sel = None
-    if neq < P:
-        rng = np.random.default_rng(seed)
-        sel = np.zeros(P, np.uint8)
-        sel[rng.choice(P, neq, replace=False)] = 1
+    sel = equation_pairs(K, neq, seed) if neq < P else None
     cl1, cl2, sh = common_lines_from_rotations(pf, rotations, max_shift, shift_step, sel, device)
     x, _ = estimate_shifts(cl1, cl2, sh, K, n_theta, tol=1e-8, maxit=100, flags=_lib.CLM_FLIP_BY_INDEX)
     eqs = shift_equations(K, n_theta, cl1.cpu().numpy(), cl2.cpu().numpy(), sh.cpu().numpy(), sel,

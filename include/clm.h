@@ -554,6 +554,44 @@ int clm_common_lines_from_rotations(clm_ctx *ctx, const clm_params *p,
                                     int32_t *d_cl2, double *d_shift1d);
 
 /*
+ * The whole cryo_estimate_shifts on host buffers
+ * (sinograms/cryo_estimate_shifts.m:1-230, less the sparse assembly, which
+ * clm_shift_equations_csc_ex with CLM_FLIP_BY_INDEX does on the host): pf
+ * (as for clm_clmatrix) and rot (3 x 3 x K doubles, MATLAB order) are
+ * uploaded, clm_common_lines_from_rotations finds the pairs' lines and 1-D
+ * shifts (sel: K(K-1)/2 host bytes, nonzero = use the pair, or null for
+ * every pair), and clm_estimate_shifts_ex solves the equations
+ * (lsqr(A, b, tol, maxit) of :228 with tol 1e-8 and maxit 100 there).
+ * Outputs on the host: cl1, cl2, shift1d (K(K-1)/2 each; -1, -1, 0 for the
+ * pairs left out) and x (2K: x[2k], x[2k+1] are the shift of image k, the
+ * rows of est_shifts = transpose(reshape(x, 2, K)), :229).
+ */
+int clm_cryo_estimate_shifts(clm_ctx *ctx, const clm_params *p, const void *pf,
+                             const double *rot, const uint8_t *sel, double tol,
+                             int32_t maxit, int32_t *cl1, int32_t *cl2,
+                             double *shift1d, double *x, clm_lsqr_stats *st);
+
+/*
+ * Number of equations cryo_estimate_shifts uses for K images
+ * (cryo_estimate_shifts.m:58-94, in the same double arithmetic): every pair,
+ * the fraction memoryfactor of them (memoryfactor <= 1), or as many as a
+ * dense K(K-1)/2 x 2K double system holds in memoryfactor megabytes
+ * (memoryfactor >= 100); at least K.  CLM_E_ARG for a memoryfactor below 0,
+ * between 1 and 100, or NaN (the reference's error, :58-60).  Host only.
+ */
+int clm_n_equations(int32_t K, double memoryfactor, int64_t *n);
+
+/*
+ * A uniform draw of min(n, K(K-1)/2) pairs without replacement, as a
+ * selection (sel: K(K-1)/2 bytes, 1 = drawn), the pairs that
+ * cryo_estimate_shifts.m:130-137 picks with randperm (MATLAB's stream is
+ * not reproduced; the draw is a function of seed).  count: pairs drawn.
+ * Host only.
+ */
+int clm_equation_pairs(int32_t K, int64_t n, uint64_t seed, uint8_t *sel,
+                       int64_t *count);
+
+/*
  * The voting synchronization matrix on the device: the consumer of the
  * common lines in the C1 ab-initio pipeline.  Replaces
  *   S = cryo_syncmatrix_vote(clstack, n_theta)

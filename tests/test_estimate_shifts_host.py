@@ -5,8 +5,6 @@
   with ``CLM_FLIP_BY_INDEX``) equal the oracle's, for all pairs and for a
   subset of them, at n_theta = 150 where the index rule and cryo_clmatrix's
   angle rule part;
-* the number of equations (cryo_estimate_shifts.m:58-94) on hand-computed
-  cases, and its argument check.
 """
 import numpy as np
 import pytest
@@ -14,7 +12,6 @@ import scipy.sparse as sp
 
 from aspire_amd import _lib
 from aspire_amd.clmatrix import shift_equations
-from aspire_amd.shifts import n_equations
 from oracle import clm_oracle as co
 from oracle import shifts_oracle as so
 
@@ -51,21 +48,3 @@ def test_equations_by_index_match_oracle(subset):
     assert rev.any()
     np.testing.assert_allclose(e0[rev], -e1[rev], rtol=0, atol=4e-16)
     assert np.array_equal(e0[~rev], e1[~rev])
-
-
-def test_n_equations():
-    # memoryfactor <= 1: the fraction of the K(K-1)/2 pairs, at least K
-    assert n_equations(10, 0.3) == 14  # ceil(90 * 0.3 / 2) = ceil(13.5)
-    assert n_equations(10, 1.0) == 45
-    assert n_equations(10, 0.01) == 10  # ceil(0.45) = 1, raised to K
-    assert n_equations(10, 0) == 10
-    # memoryfactor in MB of the dense 45 x 20 system (7200 bytes): all pairs fit
-    assert n_equations(10, 100) == 45
-    # K = 1000: 499500 pairs, 7.992e9 bytes dense; 1000 MB is 62500 pairs in
-    # exact arithmetic, but the reference's doubles (fl(1e9/7.992e9) =
-    # 0.12512512512512514, times 999000, halved: 62500.00000000001) round up
-    assert n_equations(1000, 1000) == 62501
-    assert n_equations(1000, 500) == 31251  # 31250.000000000004 likewise
-    for bad in (-0.5, 1.5, 99):
-        with pytest.raises(ValueError):
-            n_equations(10, bad)

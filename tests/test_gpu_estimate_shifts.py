@@ -174,12 +174,12 @@ def test_cryo_estimate_shifts_memoryfactor():
     assert stats["flag"] == io["flag"] and abs(stats["iterations"] - io["iterations"]) <= 1
     if stats["iterations"] == io["iterations"]:
         # 234 equations in 80 unknowns, the gauge's singular values near 0.01
-        # against 3.5 at the top: over 61 iterations the bidiagonalization
+        # against 3.5 at the top: over 60-70 iterations the bidiagonalization
         # loses orthogonality, and the two solves (their sums in different
         # orders) take slightly different paths (their running norm estimates
-        # part in the fourth digit) to the same solution, within what the
-        # stopping test (tol 1e-8) fixes of it; scipy's lsqr lands 4e-10 (of
-        # the largest entry) from the oracle here
+        # can part in the fourth digit) to the same solution, within what the
+        # stopping test (tol 1e-8) fixes of it (scipy's lsqr lands about 4e-10
+        # of the largest entry from the oracle on such a subset)
         np.testing.assert_allclose(est.ravel(), xo, rtol=0, atol=1e-7 * np.abs(xo).max())
         r = b - A @ est.ravel()
         ro = b - A @ xo

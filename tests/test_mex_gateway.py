@@ -23,8 +23,13 @@ MX_DOUBLE, MX_SINGLE = 6, 7
 
 @pytest.fixture(scope="module")
 def mex():
+    return load_shim("libclm_mex_shim.so")
+
+
+def load_shim(name):
+    """Build the gateways against the stand-in MEX API and load one of them."""
     subprocess.run(["make", "-s", "-C", SHIM], check=True)
-    L = ctypes.CDLL(os.path.join(SHIM, "libclm_mex_shim.so"))
+    L = ctypes.CDLL(os.path.join(SHIM, name))
     vp = ctypes.c_void_p
     L.shim_create.restype = vp
     L.shim_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), vp]
