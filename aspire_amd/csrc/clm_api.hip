@@ -1979,21 +1979,23 @@ int clm_n_equations(int32_t K, double memoryfactor, int64_t* n) {
 int clm_equation_pairs(int32_t K, int64_t n, uint64_t seed, uint8_t* sel, int64_t* count) {
     // n of the K(K-1)/2 pairs, every n-subset equally likely (the pairs
     // cryo_estimate_shifts.m:130-137 draws with randperm; MATLAB's stream is
-    // not reproduced): selection sampling (Knuth, TAOCP vol. 2, 3.4.2,
-    // Algorithm S) over the pairs in order, one splitmix64 draw per pair
+    // not reproduced): Floyd's sampling (J. Bentley and R. Floyd, CACM
+    // 30(9), 1987), one draw per pair taken, the selection itself as the
+    // membership set.  For j = P-n .. P-1: t uniform in [0, j]; take t, or j
+    // if t is taken already.
     if (K < 0 || n < 0) return CLM_E_ARG;
     const long long P = clm_num_pairs(K);
     if (P > 0 && !sel) return CLM_E_ARG;
     const long long want = std::min<long long>(n, P);
+    if (P > 0) std::memset(sel, 0, (size_t)P);
     uint64_t st = seed ^ 0x9e3779b97f4a7c15ull;
-    long long got = 0;
-    for (long long i = 0; i < P; i++) {
-        const double u = (double)(splitmix64(st) >> 11) * 0x1p-53;  // [0, 1)
-        const bool take = (double)(P - i) * u < (double)(want - got);
-        sel[i] = take ? 1 : 0;
-        got += take;
+    for (long long j = P - want; j < P; j++) {
+        // uniform in [0, j] by a 64 x 64 -> 128-bit product (relative bias below (j+1)/2^64)
+        const long long t = (long long)(((unsigned __int128)splitmix64(st) * (unsigned __int128)(j + 1)) >> 64);
+        if (sel[t]) sel[j] = 1;
+        else sel[t] = 1;
     }
-    if (count) *count = got;
+    if (count) *count = want;
     return CLM_OK;
 }
 
