@@ -10,6 +10,8 @@ a clean stack; ``system(..., flip_by_index=True)`` and ``lsqr``):
   column-major device tensor;
 * a pair selection: the selected pairs' outputs equal the full call's, the
   others are cl1 = cl2 = -1 and shift 0;
+* the smallest stacks (K = 2, 3) and a pair of equal rotations (Ut the
+  identity up to rounding) equal the oracle too;
 * the drop-in function: its equations equal the oracle's, its estimate the
   oracle's LSQR of them (tol 1e-8, maxit 100), and on a clean stack the
   true shifts outside the 3-dimensional gauge; with memoryfactor <= 1 the
@@ -185,3 +187,25 @@ def test_cryo_estimate_shifts_memoryfactor():
         ro = b - A @ xo
         assert abs(np.linalg.norm(r) - np.linalg.norm(ro)) <= 1e-12 * np.linalg.norm(b)
         assert np.linalg.norm(A.T @ r) <= 1e-7 * sp.linalg.norm(A) * np.linalg.norm(r)
+
+
+@pytest.mark.parametrize("K", [2, 3])
+def test_smallest_stacks_and_repeated_rotation(K):
+    # K = 2: one pair (the reference would index past its one pair, since
+    # n_equations(2, .) = 2; here every pair is used); K = 3 with image 1's
+    # rotation equal to image 0's (a degenerate pair: Ut is the identity up to
+    # rounding)
+    say("case estimate-shifts-small", K)
+    st, rot, P = stack(360)
+    rot = np.ascontiguousarray(rot[:, :, :K])
+    if K == 3:
+        rot[:, :, 1] = rot[:, :, 0]
+    pf = np.ascontiguousarray(st.pf[:, :, :K])
+    Pk, _ = co.preprocess(pf)
+    k1, k2 = co.all_pairs(K)
+    ora = so.rotation_pairs(Pk, rot, 360, 9, 1.0, k1, k2)
+    cl1, cl2, sh = common_lines_from_rotations(pf, rot, 9, 1.0)
+    assert np.array_equal(cl1.cpu().numpy(), ora["cl1"]) and np.array_equal(cl2.cpu().numpy(), ora["cl2"])
+    assert np.array_equal(sh.cpu().numpy(), ora["shift"])
+    est, eqs = cryo_estimate_shifts(pf, rot, 9, 1.0)
+    assert est.shape == (K, 2) and np.isfinite(est).all() and eqs.shape == (K * (K - 1) // 2, 2 * K + 1)
