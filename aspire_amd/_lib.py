@@ -16,6 +16,7 @@ CLM_FP32, CLM_BF16_RESCORE, CLM_FP16_RESCORE, CLM_FP64_REF = 0, 1, 2, 3
 CLM_C128, CLM_C64 = 0, 1
 CLM_R64, CLM_R32 = 0, 1
 CLM_FLIP_BY_INDEX = 1  # orientation of an equation's second line by its index
+CLM_COMPACT_RECORDS = 2  # records of the selected pairs only
 CLM_PFT_EXACT, CLM_PFT_NUFFT = 0, 1
 
 PRECISIONS = {

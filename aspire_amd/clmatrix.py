@@ -231,7 +231,9 @@ def shift_equations(K: int, n_theta: int, cl1, cl2, shift, sel=None, flags=0):
     from per-pair lines and 1-D shifts (host FP64, libclm's
     clm_shift_equations_csc_ex): one row per pair in pair order, or per
     selected pair with ``sel`` (P bytes); ``flags`` ``_lib.CLM_FLIP_BY_INDEX``
-    builds the rows as cryo_estimate_shifts does."""
+    builds the rows as cryo_estimate_shifts does, and with
+    ``_lib.CLM_COMPACT_RECORDS`` cl1, cl2 and shift hold the selected pairs'
+    records only, in pair order."""
     import scipy.sparse as sp
 
     L = _lib.load()
@@ -246,6 +248,11 @@ def shift_equations(K: int, n_theta: int, cl1, cl2, shift, sel=None, flags=0):
     shift = np.ascontiguousarray(shift, dtype=np.float64)
     vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
     nrow = P if selh is None else int(np.count_nonzero(selh))
+    nrec = nrow if flags & _lib.CLM_COMPACT_RECORDS else P
+    if selh is None and flags & _lib.CLM_COMPACT_RECORDS:
+        raise ValueError("compact records need a selection")
+    if not (cl1.shape == cl2.shape == shift.shape == (nrec,)):
+        raise ValueError(f"expected {nrec} records")
     csc = lambda *a: L.clm_shift_equations_csc_ex(int(K), int(n_theta), vp(selh), int(flags), *a)
     nnz = ctypes.c_int64(0)
     jc = np.zeros(2 * K + 2, np.int64)

@@ -2053,7 +2053,8 @@ int clm_shift_equations_csc(int32_t K, int32_t n_theta, const int32_t* cl1, cons
 int clm_shift_equations_csc_ex(int32_t K, int32_t n_theta, const uint8_t* sel, int32_t flags, const int32_t* cl1,
                                const int32_t* cl2, const double* shift1d, int64_t* jc, int64_t* ir, double* pr,
                                int64_t* nnz) {
-    if (flags & ~CLM_FLIP_BY_INDEX) return CLM_E_ARG;
+    if (flags & ~(CLM_FLIP_BY_INDEX | CLM_COMPACT_RECORDS)) return CLM_E_ARG;
+    if ((flags & CLM_COMPACT_RECORDS) && !sel && K > 1) return CLM_E_ARG;
     return shift_equations_impl(K, n_theta, sel, flags, cl1, cl2, shift1d, jc, ir, pr, nnz);
 }
 
@@ -2090,19 +2091,24 @@ int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, int32_t
         sb[cl] = std::sin(b);
         cb[cl] = std::cos(b);
     }
+    const long long P = clm_num_pairs(K);
+    const std::vector<int32_t> rank = selected_rank(P, sel);
+    auto row_of = [&](long long e) -> long long { return sel ? (long long)rank[(size_t)e] : e; };
+    // the records of pair e: at e, or at its row among the selected pairs
+    // when only those are given (CLM_COMPACT_RECORDS)
+    const bool compact = (flags & CLM_COMPACT_RECORDS) != 0;
+    auto rec_of = [&](long long e) -> long long { return compact ? row_of(e) : e; };
     // coefficient of pair e on (image k1: x,y) and (image k2: x,y)   :731-738
     auto coefs = [&](long long e, double out[4]) {
-        const int a = cl1[e], b = cl2[e];
+        const long long q = rec_of(e);
+        const int a = cl1[q], b = cl2[q];
         if (a < 0 || a > 2 * L || b < 0 || b > 2 * L) { out[0] = out[1] = out[2] = out[3] = NAN; return; }
         if (lowb[b]) { out[0] = sa[a]; out[1] = ca[a]; }
         else { out[0] = -sa[a]; out[1] = -ca[a]; }
         out[2] = -sb[b];
         out[3] = -cb[b];
     };
-    const long long P = clm_num_pairs(K);
     const long long ncol = 2 * (long long)K;
-    const std::vector<int32_t> rank = selected_rank(P, sel);
-    auto row_of = [&](long long e) -> long long { return sel ? (long long)rank[(size_t)e] : e; };
     // The nonzeros in pair order: pair e = (k1, k2) has its coefficients in
     // columns 2k1 + xy (c4[xy]) and 2k2 + xy (c4[2 + xy]) and, when its 1-D
     // shift is nonzero, an entry in column 2K (the right-hand side).  Walking
@@ -2132,7 +2138,8 @@ int shift_equations_impl(int32_t K, int32_t n_theta, const uint8_t* sel, int32_t
                     if (c4[xy] != 0.0) emit(2 * k1 + xy, row, c4[xy]);
                 for (int xy = 0; xy < 2; xy++)
                     if (c4[2 + xy] != 0.0) emit(2 * k2 + xy, row, c4[2 + xy]);
-                if (shift1d[e] != 0.0) emit(ncol, row, shift1d[e]);
+                const double b = shift1d[rec_of(e)];
+                if (b != 0.0) emit(ncol, row, b);
             }
             if (++k2 == K) {
                 k1++;

@@ -146,6 +146,8 @@ def cryo_estimate_shifts(pf, rotations, max_shift, shift_step=1.0, memoryfactor=
     (:228) on the device; shift_equations is assembled on the host (FP64).
     ``shifts_2d_ref`` and ``verbose`` feed only the reference's diagnostics
     and are accepted and ignored."""
+    import torch
+
     from .clmatrix import shift_equations
 
     shp = tuple(pf.shape)
@@ -155,6 +157,11 @@ def cryo_estimate_shifts(pf, rotations, max_shift, shift_step=1.0, memoryfactor=
     sel = equation_pairs(K, neq, seed) if neq < P else None
     cl1, cl2, sh = common_lines_from_rotations(pf, rotations, max_shift, shift_step, sel, device)
     x, _ = estimate_shifts(cl1, cl2, sh, K, n_theta, tol=1e-8, maxit=100, flags=_lib.CLM_FLIP_BY_INDEX)
-    eqs = shift_equations(K, n_theta, cl1.cpu().numpy(), cl2.cpu().numpy(), sh.cpu().numpy(), sel,
-                          _lib.CLM_FLIP_BY_INDEX)
+    flags = _lib.CLM_FLIP_BY_INDEX
+    if sel is not None:
+        # only the selected pairs' records leave the device
+        idx = torch.as_tensor(np.flatnonzero(sel), device=cl1.device)
+        cl1, cl2, sh = cl1[idx], cl2[idx], sh[idx]
+        flags |= _lib.CLM_COMPACT_RECORDS
+    eqs = shift_equations(K, n_theta, cl1.cpu().numpy(), cl2.cpu().numpy(), sh.cpu().numpy(), sel, flags)
     return x.cpu().numpy(), eqs

@@ -410,9 +410,17 @@ int clm_shift_equations_csc_subset(int32_t K, int32_t n_theta,
 #define CLM_FLIP_BY_INDEX 1
 
 /*
+ * clm_shift_equations_csc_ex flag: with a selection, cl1, cl2 and shift1d
+ * hold only the selected pairs' records, in pair order (one per row of the
+ * result), instead of K(K-1)/2 records.
+ */
+#define CLM_COMPACT_RECORDS 2
+
+/*
  * clm_shift_equations_csc with an optional selection (sel null: every
- * pair; else the rows of the selected pairs, as the _subset form) and the
- * orientation rule of the second line (flags: 0 or CLM_FLIP_BY_INDEX).
+ * pair; else the rows of the selected pairs, as the _subset form) and
+ * flags: CLM_FLIP_BY_INDEX (the orientation rule of the second line) and
+ * CLM_COMPACT_RECORDS (records of the selected pairs only; needs sel).
  */
 int clm_shift_equations_csc_ex(int32_t K, int32_t n_theta, const uint8_t *sel,
                                int32_t flags, const int32_t *cl1,

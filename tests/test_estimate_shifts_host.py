@@ -41,6 +41,14 @@ def test_equations_by_index_match_oracle(subset):
     assert eqs.shape == ref.shape == (int(keep.sum()), 2 * K + 1)
     assert np.array_equal(eqs.indptr, ref.indptr) and np.array_equal(eqs.indices, ref.indices)
     np.testing.assert_allclose(eqs.data, ref.data, rtol=0, atol=4e-16)
+    if subset:
+        # the selected pairs' records alone give the same matrix
+        c = shift_equations(K, n_theta, cl1[keep], cl2[keep], shift[keep], sel,
+                            _lib.CLM_FLIP_BY_INDEX | _lib.CLM_COMPACT_RECORDS)
+        assert np.array_equal(c.indptr, eqs.indptr) and np.array_equal(c.indices, eqs.indices)
+        assert np.array_equal(c.data, eqs.data)
+        with pytest.raises(ValueError):
+            shift_equations(K, n_theta, cl1, cl2, shift, sel, _lib.CLM_COMPACT_RECORDS)
     # the angle rule (flags 0) reverses exactly the rows with cl2 = L + 1
     e0 = shift_equations(K, n_theta, cl1, cl2, shift, sel, 0).toarray()[:, :-1]
     e1 = eqs.toarray()[:, :-1]
